@@ -1,0 +1,153 @@
+/* holo.h -- C ABI of the MI355X angular-spectrum / Gerchberg-Saxton library (libholo.so).
+ *
+ * Every entry point replaces one function of ParticleHolography.jl's src/holofunc.jl; the
+ * comment above each declaration names it (file:line). All pointers are device pointers
+ * (HIP global memory) unless stated; the caller allocates every input and output. Calls are
+ * asynchronous and ordered on the given `stream` (a hipStream_t, NULL = the null stream).
+ *
+ * Layout: a plane of nx x ny values is stored x-contiguous: element (i, j) at j*nx + i. This
+ * is the reference's column-major CuArray{T,2} of size (nx, ny) (Julia dim 1 = x). Complex
+ * values are interleaved (re, im) float pairs (ComplexF32). Volumes are nx*ny*slices with
+ * the slice index slowest (Julia CuArray{T,3} of size (nx, ny, slices)).
+ *
+ * Supported FFT lengths (per axis): 2^a, 3*2^a and 5*2^a, up to 4096 (holo_supported_length).
+ *
+ * Return value: HOLO_OK (0) or an error code; holo_last_error() gives the message of the
+ * calling thread's last failure. A handle is not thread safe: use one handle per host thread.
+ */
+#ifndef HOLO_H
+#define HOLO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct holo_context* holo_handle;
+
+enum holo_status {
+  HOLO_OK = 0,
+  HOLO_ERR_ARG = 1,     /* bad argument / size mismatch (the reference's AssertionError) */
+  HOLO_ERR_SIZE = 2,    /* FFT length not supported */
+  HOLO_ERR_TYPE = 3,    /* unsupported element type (the reference's ArgumentError) */
+  HOLO_ERR_HIP = 4,     /* HIP runtime error */
+  HOLO_ERR_NOMEM = 5,   /* device allocation failed */
+  HOLO_ERR_BUDGET = 6   /* request exceeds the configured device-memory budget */
+};
+
+/* Element types of volume outputs (return_type in holofunc.jl:158-166). */
+enum holo_dtype {
+  HOLO_N0F8 = 0, /* FixedPointNumbers N0f8: uint8 bit pattern, value u/255 */
+  HOLO_F16 = 1,
+  HOLO_F32 = 2,
+  HOLO_F64 = 3,
+  HOLO_C64 = 4   /* ComplexF32 (complex volume, holofunc.jl:216-232) */
+};
+
+/* Precision of a transfer function built by holo_transfer. */
+enum holo_tf_precision {
+  HOLO_TF_REFERENCE = 0, /* the reference's arithmetic: Float32 sqrt of d, FP64 exp -> CF32 */
+  HOLO_TF_FP64_D = 1,    /* FP64 sqrt of the given d values, phase wrapped mod 2pi in FP64 */
+  HOLO_TF_FP64 = 2       /* d recomputed in FP64 from (wavlen, dx), phase wrapped in FP64 */
+};
+
+/* A transfer function as consumed by the reconstruction / propagation entry points.
+ * generated = 1: H(z) is generated on the device in FP64-wrapped form from (z, wavlen, dx)
+ *                (no array read); `data` is ignored.
+ * generated = 0: `data` is a centred (fftshifted) ComplexF32 nx*ny array, as returned by
+ *                holo_transfer / the reference's cu_transfer (types.jl:25-27). */
+typedef struct holo_tf {
+  const void* data;
+  double z;
+  double wavlen;
+  double dx;
+  int generated;
+} holo_tf;
+
+/* ---- lifecycle -------------------------------------------------------------------------- */
+int holo_create(holo_handle* out, int device);
+int holo_destroy(holo_handle h);
+const char* holo_last_error(void);
+int holo_version(void);
+int holo_supported_length(int n);
+/* Tuning knobs (0 = library default): slices per batch, slices per workgroup in the column
+ * pass and in the row pass, columns per workgroup (power of two), rows per workgroup. */
+int holo_set_tuning(holo_handle h, int batch, int spg_cols, int spg_rows, int cols_per_wg,
+                    int rows_per_wg);
+
+/* ---- A2: cu_transfer_sqrt_arr (holofunc.jl:8-15, 30-36) --------------------------------- */
+/* d[x,y] = 1 - ((x-nx/2-1)wavlen/nx/dx)^2 - ((y-ny/2-1)wavlen/ny/dx)^2 (1-based x, y), FP64
+ * arithmetic in the reference's order, stored as Float32 (dtype HOLO_F32, bit-identical to the
+ * reference) or Float64 (HOLO_F64). */
+int holo_transfer_sqrt(holo_handle h, void* stream, int nx, int ny, double wavlen, double dx,
+                       int dtype, void* out);
+
+/* ---- A3: cu_transfer (holofunc.jl:38-45, 61-67) ----------------------------------------- */
+/* H = exp(2 pi i z / wavlen * sqrt(d)) as a centred ComplexF32 array. `d_sqr` (F32 or F64,
+ * d_dtype) is read for precisions HOLO_TF_REFERENCE and HOLO_TF_FP64_D; HOLO_TF_FP64 uses
+ * (wavlen, dx) only and may pass d_sqr = NULL. */
+int holo_transfer(holo_handle h, void* stream, int nx, int ny, double z, double wavlen,
+                  const void* d_sqr, int d_dtype, int precision, double dx, void* out);
+
+/* ---- A4: cu_gabor_wavefront (holofunc.jl:80-82) ----------------------------------------- */
+int holo_gabor_wavefront(holo_handle h, void* stream, int nx, int ny, const float* holo,
+                         void* out);
+
+/* ---- A6: cu_get_reconst_xyprojection (holofunc.jl:248-266) ------------------------------ */
+/* proj = min over k = 0..slices-1 of |IFFT2(FFT2(wavefront) .* H_k)|^2 with H_k the transfer
+ * to front.z + k*dz.z. With two generated transfers H_k is evaluated per slice in FP64-wrapped
+ * form; with two arrays the reference's ComplexF32 running product H0 .* Hdz^k is used. */
+int holo_reconst_xyprojection(holo_handle h, void* stream, int nx, int ny,
+                              const void* wavefront, const holo_tf* front,
+                              const holo_tf* dz, int slices, float* proj);
+
+/* Same, starting from a real intensity hologram: the wavefront sqrt(holo) is fused into the
+ * first FFT pass (cu_gabor_wavefront followed by cu_get_reconst_xyprojection). */
+int holo_reconst_xyprojection_holo(holo_handle h, void* stream, int nx, int ny,
+                                   const float* holo, const holo_tf* front,
+                                   const holo_tf* dz, int slices, float* proj);
+
+/* ---- A7 / A8 / A9: cu_get_reconst_vol, cu_get_reconst_complex_vol,
+ *      cu_get_reconst_vol_and_xyprojection (holofunc.jl:184-200, 216-232, 300-321) -------- */
+/* vol (nx*ny*slices of out_dtype). If proj != NULL (A9, Float32 nx*ny) it receives the min over
+ * slices of the stored values (N0f8 as u/255). HOLO_C64 stores psi_k unclamped (A8) and
+ * requires proj == NULL. */
+int holo_reconst_vol(holo_handle h, void* stream, int nx, int ny, const void* wavefront,
+                     const holo_tf* front, const holo_tf* dz, int slices,
+                     int out_dtype, void* vol, float* proj);
+
+/* ---- f2: cu_2d_pad and cu_get_reconst_vol_and_xyprojection_padded
+ *      (holofunc.jl:139-149, 339-364) ------------------------------------------------------- */
+/* out (2nx x 2ny ComplexF32) = mean(in) everywhere, `in` copied at offset (nx/2, ny/2). */
+int holo_2d_pad(holo_handle h, void* stream, int nx, int ny, const void* in, void* out);
+/* Transfers are 2nx x 2ny; vol is N0f8 nx*ny*slices (the centre window), proj Float32 nx*ny. */
+int holo_reconst_vol_and_xyprojection_padded(holo_handle h, void* stream, int nx, int ny,
+                                             const void* wavefront, const holo_tf* front,
+                                             const holo_tf* dz, int slices,
+                                             uint8_t* vol, float* proj);
+
+/* ---- A10: cu_phase_retrieval_holo (holofunc.jl:100-129) --------------------------------- */
+/* Gerchberg-Saxton: psi1 = sqrt(I1); `iters` times: psi2 = P_fwd psi1, psi2 <- sqrt(I2) psi2/|psi2|,
+ * psi1 = P_bwd psi2, psi1 <- sqrt(I1) psi1/|psi1|. out (ComplexF32 nx*ny) = psi1. With
+ * use_graph != 0 one iteration is captured as a hipGraph and replayed. */
+int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const float* holo1,
+                         const float* holo2, const holo_tf* fwd, const holo_tf* bwd,
+                         int iters, void* out, int use_graph);
+
+/* ---- A11: cu_asm_prop! (holofunc.jl:383-388) -------------------------------------------- */
+/* out = IFFT2(FFT2(in) .* H(tf)); out may alias in. */
+int holo_asm_prop(holo_handle h, void* stream, int nx, int ny, void* out, const void* in,
+                  const holo_tf* tf);
+
+/* ---- A12: CUFFT fft / ifft (call sites in holofunc.jl) ---------------------------------- */
+/* 2-D C2C FFT, forward unnormalized, inverse scaled by 1/(nx*ny) (AbstractFFTs convention).
+ * out may alias in. */
+int holo_fft2(holo_handle h, void* stream, int nx, int ny, const void* in, void* out,
+              int inverse);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HOLO_H */

@@ -1,0 +1,311 @@
+"""CPU oracle for the angular-spectrum / Gerchberg-Saxton path (test infrastructure only).
+
+This module is the checker. Only tests, the smoke check and the CPU-baseline leg of the
+benchmark may import it; the product path never calls it.
+
+Array convention: numpy arrays are C-ordered with shape ``(ny, nx)``. Element ``[j, i]``
+is the reference's Julia element ``[i+1, j+1]`` (Julia dim 1 = ``x`` = the contiguous axis),
+so a numpy buffer and the reference's column-major ``CuArray`` have the same memory layout.
+
+Two arithmetic modes are restated:
+
+* ``exact`` -- FP64 everywhere: FP64 ``d``, FP64 ``sqrt``, FP64 ``exp``, FP64 FFTs. This is
+  the mathematics of the method (docs/src/whats_inline_holography.md:11-15, 38-42, 60-89).
+* ``reference`` -- the reference's own arithmetic for the transfer functions: Float32 ``d``
+  (holofunc.jl:8-15), Float32 ``sqrt`` then FP64 ``exp`` rounded to ComplexF32
+  (holofunc.jl:38-45), and a ComplexF32 running product ``F .*= Hdz`` (holofunc.jl:259-263).
+  FFTs are done in FP64 on the rounded ComplexF32 values (cuFFT's own rounding is not
+  restated; it is a closed library).
+
+Parity status of every function is listed in ``PARITY`` at the bottom of this file.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+TWO_PI = 2.0 * np.pi  # Julia ``2im * pi`` -> ComplexF64(0, 2.0*pi)
+
+
+# --------------------------------------------------------------------------------------
+# A2: square-root part  (holofunc.jl:8-15, 30-36)
+# --------------------------------------------------------------------------------------
+def axis_term(n: int, wavlen: float, dx: float) -> np.ndarray:
+    """((x - n/2 - 1.0) * wavlen / n / dx)^2 for 1-based x = 1..n, in FP64 with the
+    reference's operation order (holofunc.jl:12). Index is the *centred* (fftshifted) one."""
+    x = np.arange(1, n + 1, dtype=np.float64)
+    t = (x - n / 2 - 1.0) * wavlen / n / dx
+    return t * t  # Julia literal ``^2`` lowers to ``t*t``
+
+
+def transfer_sqrt_arr(nx: int, wavlen: float, dx: float, ny: int | None = None,
+                      dtype=np.float32) -> np.ndarray:
+    """d[x,y] = 1 - ax - ay, evaluated left to right in FP64 and stored as ``dtype``
+    (holofunc.jl:12: ``1.0 - (...)^2 - (...)^2``). Centred layout, shape (ny, nx)."""
+    ny = nx if ny is None else ny
+    ax = axis_term(nx, wavlen, dx)
+    ay = axis_term(ny, wavlen, dx)
+    d = (1.0 - ax[None, :]) - ay[:, None]
+    return d.astype(dtype)
+
+
+# --------------------------------------------------------------------------------------
+# A3: transfer function  (holofunc.jl:38-45, 61-67)
+# --------------------------------------------------------------------------------------
+def transfer_reference(z: float, wavlen: float, d32: np.ndarray) -> np.ndarray:
+    """Reference arithmetic: theta = ((2pi*z)/wavlen) * Float64(sqrtf(d32)); exp in FP64;
+    rounded to ComplexF32 (holofunc.jl:42). Negative d gives NaN, as sqrtf does."""
+    with np.errstate(invalid="ignore"):
+        s32 = np.sqrt(d32.astype(np.float32))  # Float32 sqrt, correctly rounded
+    theta = ((TWO_PI * z) / wavlen) * s32.astype(np.float64)
+    return (np.cos(theta) + 1j * np.sin(theta)).astype(np.complex64)
+
+
+def transfer_exact(z: float, wavlen: float, dx: float, nx: int, ny: int | None = None) -> np.ndarray:
+    """Exact FP64 transfer function exp(i 2pi z/wavlen sqrt(d)) on the centred grid, complex128.
+    Evanescent components (d < 0) are set to 0 (the build's documented choice; the
+    reference yields NaN there)."""
+    d = transfer_sqrt_arr(nx, wavlen, dx, ny, dtype=np.float64)
+    s = np.sqrt(np.maximum(d, 0.0))
+    theta = (TWO_PI * z / wavlen) * s
+    h = np.cos(theta) + 1j * np.sin(theta)
+    h[d < 0] = 0.0
+    return h
+
+
+def natural(h_centred: np.ndarray) -> np.ndarray:
+    """ifftshift: centred (fftshifted) layout -> natural FFT order. AbstractFFTs and numpy
+    agree (circshift by -div(n,2) per axis)."""
+    return np.fft.ifftshift(h_centred)
+
+
+# --------------------------------------------------------------------------------------
+# A4, A12: wavefront and FFT conventions
+# --------------------------------------------------------------------------------------
+def gabor_wavefront(holo: np.ndarray) -> np.ndarray:
+    """ComplexF32(sqrt(holo) + 0im)  (holofunc.jl:80-82); sqrt in Float32."""
+    with np.errstate(invalid="ignore"):
+        return np.sqrt(holo.astype(np.float32)).astype(np.complex64)
+
+
+def fft2(x):
+    return np.fft.fft2(x.astype(np.complex128))
+
+
+def ifft2(x):
+    """Inverse FFT scaled by 1/(nx*ny) (AbstractFFTs convention)."""
+    return np.fft.ifft2(x.astype(np.complex128))
+
+
+def propagate(psi: np.ndarray, h_centred: np.ndarray) -> np.ndarray:
+    """IFFT2(ifftshift(fftshift(FFT2(psi)) .* H)) = IFFT2(FFT2(psi) .* ifftshift(H))
+    (holofunc.jl:114, 121, 386). FP64."""
+    return ifft2(fft2(psi) * natural(h_centred))
+
+
+def asm_prop(psi: np.ndarray, z: float, wavlen: float, dx: float, mode: str = "exact") -> np.ndarray:
+    """cu_asm_prop! (holofunc.jl:383-388): builds H(z) and propagates once."""
+    ny, nx = psi.shape
+    if mode == "exact":
+        h = transfer_exact(z, wavlen, dx, nx, ny)
+    else:
+        h = transfer_reference(z, wavlen, transfer_sqrt_arr(nx, wavlen, dx, ny))
+    return propagate(psi, h)
+
+
+# --------------------------------------------------------------------------------------
+# A6-A9: slice reconstruction  (holofunc.jl:184-200, 216-232, 248-266, 268-282, 300-321)
+# --------------------------------------------------------------------------------------
+def _slice_fields(wavefront, slices, mode, z_front=None, z_dz=None, wavlen=None, dx=None,
+                  h_front=None, h_dz=None):
+    """Yields psi_k for k = 0..slices-1, psi_k at z_front + k*z_dz.
+
+    mode 'exact': H generated in FP64 for each slice (complex128 spectrum, FP64 FFTs).
+    mode 'reference': ComplexF32 spectrum ``fftshift(fft(psi)) .* H0`` then ``.*= Hdz``
+    per slice in ComplexF32 (holofunc.jl:254-260), FP64 inverse FFT of the rounded values.
+    """
+    ny, nx = wavefront.shape
+    if mode == "exact":
+        F = fft2(wavefront)
+        d = transfer_sqrt_arr(nx, wavlen, dx, ny, dtype=np.float64)
+        s = natural(np.sqrt(np.maximum(d, 0.0)))
+        evan = natural(d < 0)
+        for k in range(slices):
+            z = z_front + k * z_dz
+            theta = (TWO_PI * z / wavlen) * s
+            h = np.cos(theta) + 1j * np.sin(theta)
+            h[evan] = 0.0
+            yield ifft2(F * h)
+    elif mode == "reference":
+        F = np.fft.fftshift(fft2(wavefront)).astype(np.complex64)
+        F = (F * h_front.astype(np.complex64)).astype(np.complex64)
+        hdz = h_dz.astype(np.complex64)
+        for k in range(slices):
+            if k > 0:
+                F = (F * hdz).astype(np.complex64)
+            yield ifft2(np.fft.ifftshift(F))
+    else:
+        raise ValueError(mode)
+
+
+def reconst_xyprojection(wavefront, slices, mode="exact", **kw) -> np.ndarray:
+    """min over slices of abs2(psi_k), unclamped, returned as Float32 (holofunc.jl:248-266).
+    ``CUDA.min.(proj, projtmp)`` runs CUDA.jl's device method of ``min`` for Float32, which
+    calls libdevice ``fminf``: a NaN operand is ignored (np.fmin has the same rule)."""
+    proj = None
+    for psi in _slice_fields(wavefront, slices, mode, **kw):
+        inten = (psi.real ** 2 + psi.imag ** 2)
+        proj = inten if proj is None else np.fmin(proj, inten)
+    return proj.astype(np.float32)
+
+
+def n0f8(v: np.ndarray) -> np.ndarray:
+    """reinterpret(N0f8, round(UInt8, clamp(v,0,1)*255)) with ties-to-even
+    (holofunc.jl:154-156). ``v`` is promoted to FP64 first (clamp with 0.0/1.0 literals)."""
+    c = np.clip(v.astype(np.float64), 0.0, 1.0)
+    return np.rint(c * 255.0).astype(np.uint8)
+
+
+def store_value(inten_f32: np.ndarray, out_dtype) -> np.ndarray:
+    """_ifft_and_abs epilogue (holofunc.jl:158-166): abs2 (Float32) -> clamp(.,0.0,1.0) in FP64
+    -> return_type. ``out_dtype`` in {np.uint8 (N0f8), np.float16, np.float32, np.float64}."""
+    if out_dtype == np.uint8:
+        return n0f8(inten_f32)
+    c = np.clip(inten_f32.astype(np.float64), 0.0, 1.0)
+    return c.astype(out_dtype)
+
+
+def reconst_vol(wavefront, slices, out_dtype=np.uint8, mode="exact", **kw) -> np.ndarray:
+    """cu_get_reconst_vol (holofunc.jl:184-200). Shape (slices, ny, nx) = Julia (nx, ny, slices)."""
+    out = []
+    for psi in _slice_fields(wavefront, slices, mode, **kw):
+        inten = (psi.real ** 2 + psi.imag ** 2).astype(np.float32)
+        out.append(store_value(inten, out_dtype))
+    return np.stack(out)
+
+
+def reconst_complex_vol(wavefront, slices, mode="exact", **kw) -> np.ndarray:
+    """cu_get_reconst_complex_vol (holofunc.jl:216-232): psi_k, unclamped."""
+    return np.stack([psi for psi in _slice_fields(wavefront, slices, mode, **kw)])
+
+
+def stored_to_f32(vol: np.ndarray) -> np.ndarray:
+    """Float32 view of stored values: N0f8 -> u/255 (correctly rounded), floats -> Float32."""
+    if vol.dtype == np.uint8:
+        return (vol.astype(np.float32) / np.float32(255.0)).astype(np.float32)
+    return vol.astype(np.float32)
+
+
+def xy_projection_from_vol(vol: np.ndarray) -> np.ndarray:
+    """_cu_get_xy_projection_from_vol! (holofunc.jl:268-282): ``val < min ? val : min`` over z,
+    starting from slice 1, so a NaN is ignored after the first slice; stored into Float32."""
+    vals = stored_to_f32(vol)
+    m = vals[0].copy()
+    for k in range(1, vals.shape[0]):
+        v = vals[k]
+        m = np.where(v < m, v, m)
+    return m
+
+
+def reconst_vol_and_xyprojection(wavefront, slices, out_dtype=np.uint8, mode="exact", **kw):
+    """cu_get_reconst_vol_and_xyprojection (holofunc.jl:300-321)."""
+    vol = reconst_vol(wavefront, slices, out_dtype, mode, **kw)
+    return vol, xy_projection_from_vol(vol)
+
+
+# --------------------------------------------------------------------------------------
+# f2: padding and the padded reconstruction  (holofunc.jl:139-149, 339-364)
+# --------------------------------------------------------------------------------------
+def pad2(a: np.ndarray) -> np.ndarray:
+    """cu_2d_pad: 2N x 2N filled with ComplexF32(mean(a)), a copied at offset N/2 (0-based
+    ``outlen÷4``) in both axes (holofunc.jl:139-149)."""
+    ny, nx = a.shape
+    m = np.complex64(a.astype(np.complex128).mean())
+    out = np.full((2 * ny, 2 * nx), m, dtype=np.complex64)
+    oy, ox = (2 * ny) // 4, (2 * nx) // 4
+    out[oy:oy + ny, ox:ox + nx] = a.astype(np.complex64)
+    return out
+
+
+def reconst_vol_and_xyprojection_padded(wavefront, slices, mode="exact", **kw):
+    """Padded variant: reconstruct at 2N, keep the centre window [N/2, 3N/2) of each slice as
+    N0f8 whatever return_type is (holofunc.jl:344-356), projection from the stored values."""
+    ny, nx = wavefront.shape
+    padded = pad2(wavefront)
+    vols = []
+    oy, ox = ny // 2, nx // 2
+    for psi in _slice_fields(padded, slices, mode, **kw):
+        inten = (psi.real ** 2 + psi.imag ** 2).astype(np.float32)
+        vols.append(n0f8(inten[oy:oy + ny, ox:ox + nx]))
+    vol = np.stack(vols)
+    return vol, xy_projection_from_vol(vol)
+
+
+# --------------------------------------------------------------------------------------
+# A10: Gerchberg-Saxton  (holofunc.jl:100-129)
+# --------------------------------------------------------------------------------------
+def phase_retrieval(holo1, holo2, iters, h_fwd=None, h_bwd=None, *, z=None, wavlen=None,
+                    dx=None) -> np.ndarray:
+    """GS in FP64: psi1 = sqrt(I1); repeat: psi2 = P+(psi1); psi2 = sqrt(I2) e^{i angle psi2};
+    psi1 = P-(psi2); psi1 = sqrt(I1) e^{i angle psi1}. angle(0) = 0. Transfer functions are the
+    given centred arrays (h_fwd / h_bwd, used at FP64) or, with ``z``, exact H(+z) / H(-z).
+    The angle is taken of the spatial field (holofunc.jl:115, 122)."""
+    ny, nx = holo1.shape
+    if z is not None:
+        hf = natural(transfer_exact(z, wavlen, dx, nx, ny))
+        hb = natural(transfer_exact(-z, wavlen, dx, nx, ny))
+    else:
+        hf = natural(h_fwd.astype(np.complex128))
+        hb = natural(h_bwd.astype(np.complex128))
+    with np.errstate(invalid="ignore"):
+        a1 = np.sqrt(holo1.astype(np.float32)).astype(np.float64)
+        a2 = np.sqrt(holo2.astype(np.float32)).astype(np.float64)
+    psi1 = a1.astype(np.complex128)
+    for _ in range(iters):
+        psi2 = ifft2(fft2(psi1) * hf)
+        psi2 = a2 * np.exp(1j * np.angle(psi2))
+        psi1 = ifft2(fft2(psi2) * hb)
+        psi1 = a1 * np.exp(1j * np.angle(psi1))
+    return psi1
+
+
+def gs_error(psi1, holo2, h_fwd_nat) -> float:
+    """Fienup error-reduction error at plane 2: || |P+(psi1)| - sqrt(I2) ||_2 (used for the
+    non-increase invariant, Fienup, Appl. Opt. 21, 2758 (1982))."""
+    a2 = np.sqrt(holo2.astype(np.float64))
+    psi2 = ifft2(fft2(psi1) * h_fwd_nat)
+    return float(np.linalg.norm(np.abs(psi2) - a2))
+
+
+# --------------------------------------------------------------------------------------
+# metrics
+# --------------------------------------------------------------------------------------
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, dtype=np.complex128 if np.iscomplexobj(a) else np.float64)
+    b = np.asarray(b, dtype=a.dtype)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def weighted_phase_rms(psi, psi_ref, weight=None) -> float:
+    """sqrt(sum A^2 wrap(phi - phi_ref)^2 / sum A^2), A = |psi_ref| unless given."""
+    w = np.abs(psi_ref) ** 2 if weight is None else np.asarray(weight, np.float64) ** 2
+    dphi = np.angle(psi * np.conj(psi_ref))  # wrapped difference in (-pi, pi]
+    return float(np.sqrt(np.sum(w * dphi ** 2) / np.sum(w)))
+
+
+# --------------------------------------------------------------------------------------
+# Which check pins which function (see tests/test_oracle.py)
+# --------------------------------------------------------------------------------------
+PARITY = {
+    "transfer_sqrt_arr": "closed form: brute-force scalar re-evaluation + centre value d=1 at x=n/2+1",
+    "transfer_reference": "H(0)=1, |H|=1, H(a)H(b)=H(a+b) to F32 rounding, phase vs exact <= table bound",
+    "transfer_exact": "plane-wave eigenfunction, composition, O(N^4) brute-force DFT propagation",
+    "propagate/asm_prop": "constant field, single-frequency grating closed form, delta<->constant, "
+                          "reversibility, Parseval, brute-force DFT at 8^2/16^2",
+    "reconst_xyprojection": "constant field -> 1, grating closed-form min over z, running-product "
+                            "identity (reference mode vs exact), brute-force DFT",
+    "reconst_vol/complex_vol/vol_and_xyprojection": "per-slice equal to propagate(); N0f8 rounding "
+                                                    "ties; NaN rule of the projection kernel",
+    "pad2/padded": "cu_2d_pad known-answer test (runtests.jl:362-368)",
+    "phase_retrieval": "plane-wave fixed point, |psi1| == sqrt(I1), error-reduction non-increase, "
+                       "brute-force DFT GS iteration at 8^2",
+}

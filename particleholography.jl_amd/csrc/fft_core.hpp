@@ -1,0 +1,306 @@
+// Register/LDS Stockham FFT engine for CDNA4 (gfx950), complex FP32.
+//
+// A line of length L is owned by T = L/E threads of one workgroup; thread t holds the
+// elements n = t + T*e (e = 0..E-1) in registers. Every radix-R stage performs E/R
+// butterflies per thread in registers. Between stages the values go through LDS once
+// (write at the Stockham autosort position, barrier, read back in the t + T*e pattern), so a
+// plan with S stages costs S-1 LDS exchanges. The first stage reads its inputs in the
+// t + T*e pattern and the last stage leaves its outputs in the same pattern, which lets a
+// kernel load from global memory straight into the first stage, apply an elementwise
+// operation to the last stage's outputs in registers, and even chain a second FFT without an
+// extra LDS round trip.
+//
+// Stockham autosort recursion (Van Loan, "Computational Frameworks for the FFT", 1992,
+// section 1.7): stage with radix R and span p (product of earlier radices) takes butterfly b
+// (0 <= b < L/R) with inputs x[b + m*L/R], k = b mod p, multiplies input m by w^(m*k),
+// w = exp(-+2 pi i/(p*R)), does an R-point DFT and writes output m to
+// y[(b - k)*R + k + m*p]. After the last stage the output is in natural order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace holo {
+
+constexpr int kMaxStages = 12;
+
+struct StagePlan {
+  int R;            // radix
+  int p;            // span: product of the radices of earlier stages
+  uint32_t pmagic;  // ceil(2^32 / p) for p >= 2: b / p == umulhi(b, pmagic) for b < 2^16
+  int twoff;        // offset (complex elements) of this stage's twiddles in the table
+};
+
+struct LinePlan {
+  int L;   // line length
+  int T;   // threads per line (L / E)
+  int ns;  // number of stages
+  StagePlan st[kMaxStages];
+};
+
+// ------------------------------------------------------------------------------------------
+// complex helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulconj(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// multiply by exp(DIR * i pi / 2): DIR = -1 -> -i, DIR = +1 -> +i
+template <int DIR>
+__device__ __forceinline__ float2 mul_w4(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// ------------------------------------------------------------------------------------------
+// compile-time twiddles exp(DIR * 2 pi i m / R), exact at multiples of pi/2
+// ------------------------------------------------------------------------------------------
+namespace cx {
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double sin_taylor(double x) {
+  double t = x, s = x;
+  for (int k = 1; k < 16; ++k) { t *= -x * x / ((2.0 * k) * (2.0 * k + 1.0)); s += t; }
+  return s;
+}
+constexpr double cos_taylor(double x) {
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k < 16; ++k) { t *= -x * x / ((2.0 * k - 1.0) * (2.0 * k)); s += t; }
+  return s;
+}
+}  // namespace cx
+
+template <int R, int DIR>
+struct TwTab {
+  float c[R];
+  float s[R];
+  constexpr TwTab() : c(), s() {
+    for (int m = 0; m < R; ++m) {
+      // angle = 2 pi m / R = (pi/2) * q + 2 pi f with q = round(4m/R), |f| <= 1/8
+      const int q = (8 * m + R) / (2 * R);  // round(4m/R)
+      const double f = (4.0 * m - q * R) / (4.0 * R);
+      const double x = 2.0 * cx::kPi * f;
+      double cc = cx::cos_taylor(x), ss = cx::sin_taylor(x);
+      if (4 * m - q * R == 0) { cc = 1.0; ss = 0.0; }
+      double cr = cc, sr = ss;  // rotate by q quarter turns
+      switch (q & 3) {
+        case 0: cr = cc; sr = ss; break;
+        case 1: cr = -ss; sr = cc; break;
+        case 2: cr = -cc; sr = -ss; break;
+        default: cr = ss; sr = -cc; break;
+      }
+      c[m] = static_cast<float>(cr);
+      s[m] = static_cast<float>(DIR < 0 ? -sr : sr);
+    }
+  }
+};
+template <int R, int DIR>
+struct Tw {
+  static constexpr TwTab<R, DIR> t{};
+};
+
+// multiply by exp(DIR * 2 pi i e / R); e is a constant after unrolling
+template <int R, int DIR>
+__device__ __forceinline__ float2 twmul(float2 v, int e) {
+  e %= R;
+  if (e == 0) return v;
+  if (4 * e == R) return mul_w4<DIR>(v);
+  if (2 * e == R) return make_float2(-v.x, -v.y);
+  if (4 * e == 3 * R) return mul_w4<-DIR>(v);
+  return cmul(v, make_float2(Tw<R, DIR>::t.c[e], Tw<R, DIR>::t.s[e]));
+}
+
+// ------------------------------------------------------------------------------------------
+// small DFTs in registers (in place, natural-order output)
+// ------------------------------------------------------------------------------------------
+template <int R, int DIR>
+struct DFT;
+
+template <int DIR>
+struct DFT<1, DIR> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <int DIR>
+struct DFT<2, DIR> {
+  __device__ __forceinline__ static void run(float2* x) {
+    const float2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  }
+};
+
+template <int DIR>
+struct DFT<3, DIR> {
+  __device__ __forceinline__ static void run(float2* x) {
+    constexpr float kC = -0.5f;
+    constexpr float kS = 0.866025403784438646763723f;  // sin(2 pi / 3)
+    const float2 s = cadd(x[1], x[2]);
+    const float2 d = csub(x[1], x[2]);
+    const float2 m = make_float2(x[0].x + kC * s.x, x[0].y + kC * s.y);
+    // DIR * i * kS * d
+    const float2 r = make_float2(-DIR * kS * d.y, DIR * kS * d.x);
+    x[0] = cadd(x[0], s);
+    x[1] = cadd(m, r);
+    x[2] = csub(m, r);
+  }
+};
+
+template <int DIR>
+struct DFT<4, DIR> {
+  __device__ __forceinline__ static void run(float2* x) {
+    const float2 t0 = cadd(x[0], x[2]);
+    const float2 t1 = csub(x[0], x[2]);
+    const float2 t2 = cadd(x[1], x[3]);
+    const float2 t3 = mul_w4<DIR>(csub(x[1], x[3]));
+    x[0] = cadd(t0, t2);
+    x[2] = csub(t0, t2);
+    x[1] = cadd(t1, t3);
+    x[3] = csub(t1, t3);
+  }
+};
+
+template <int DIR>
+struct DFT<5, DIR> {
+  __device__ __forceinline__ static void run(float2* x) {
+    constexpr float c1 = 0.309016994374947424102293f;   // cos(2 pi / 5)
+    constexpr float c2 = -0.809016994374947424102293f;  // cos(4 pi / 5)
+    constexpr float s1 = 0.951056516295153572116439f;   // sin(2 pi / 5)
+    constexpr float s2 = 0.587785252292473129168706f;   // sin(4 pi / 5)
+    const float2 a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+    const float2 a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+    const float2 x0 = x[0];
+    const float2 m1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+    const float2 m2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+    const float2 n1 = make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+    const float2 n2 = make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+    // DIR * i * n
+    const float2 r1 = make_float2(-DIR * n1.y, DIR * n1.x);
+    const float2 r2 = make_float2(-DIR * n2.y, DIR * n2.x);
+    x[0] = make_float2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+    x[1] = cadd(m1, r1);
+    x[4] = csub(m1, r1);
+    x[2] = cadd(m2, r2);
+    x[3] = csub(m2, r2);
+  }
+};
+
+// R = A * B Cooley-Tukey in registers: n = A*n2 + n1, k = B*k1 + k2.
+template <int A, int B, int DIR>
+__device__ __forceinline__ void dft_composite(float2* x) {
+  constexpr int R = A * B;
+  float2 y[R];
+#pragma unroll
+  for (int n1 = 0; n1 < A; ++n1) {
+    float2 t[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) t[n2] = x[A * n2 + n1];
+    DFT<B, DIR>::run(t);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) y[n1 * B + k2] = twmul<R, DIR>(t[k2], n1 * k2);
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) {
+    float2 t[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) t[n1] = y[n1 * B + k2];
+    DFT<A, DIR>::run(t);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) x[B * k1 + k2] = t[k1];
+  }
+}
+
+template <int DIR>
+struct DFT<8, DIR> {
+  __device__ __forceinline__ static void run(float2* x) { dft_composite<2, 4, DIR>(x); }
+};
+template <int DIR>
+struct DFT<16, DIR> {
+  __device__ __forceinline__ static void run(float2* x) { dft_composite<4, 4, DIR>(x); }
+};
+
+// ------------------------------------------------------------------------------------------
+// LDS addressing: one padding slot every 16 elements breaks the power-of-two strides of the
+// autosort writes across banks.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lds_pad(int n) { return n + (n >> 4); }
+__host__ __device__ constexpr int lds_padded_len(int L) { return L + (L >> 4) + 1; }
+
+// One Stockham stage: E/R butterflies per thread, outputs to LDS unless this is the last.
+template <int R, int E, int DIR>
+__device__ __forceinline__ void fft_stage(float2 (&v)[E], const int t, const int T,
+                                          const StagePlan sp, const float2* __restrict__ tw,
+                                          float2* __restrict__ lds, const int es, const bool last) {
+  static_assert(E % R == 0, "radix must divide the per-thread element count");
+  constexpr int Q = E / R;
+  int base[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    float2 x[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) x[m] = v[q + Q * m];
+    const int b = t + T * q;
+    if (sp.p > 1) {
+      const int j = static_cast<int>(__umulhi(static_cast<uint32_t>(b), sp.pmagic));
+      const int k = b - j * sp.p;
+      const float2* __restrict__ w = tw + sp.twoff + k;
+#pragma unroll
+      for (int m = 1; m < R; ++m) {
+        const float2 wm = w[(m - 1) * sp.p];
+        x[m] = DIR < 0 ? cmul(x[m], wm) : cmulconj(x[m], wm);
+      }
+      base[q] = j * sp.p * R + k;
+    } else {
+      base[q] = b * R;
+    }
+    DFT<R, DIR>::run(x);
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[q + Q * m] = x[m];
+  }
+  if (!last) {
+    __syncthreads();  // every thread has read the previous stage's LDS contents
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int m = 0; m < R; ++m) lds[lds_pad(base[q] + m * sp.p) * es] = v[q + Q * m];
+    __syncthreads();
+  }
+}
+
+template <int R, int E, int DIR>
+__device__ __forceinline__ void fft_stage_if(float2 (&v)[E], int t, int T, const StagePlan& sp,
+                                             const float2* __restrict__ tw, float2* __restrict__ lds,
+                                             int es, bool last) {
+  if constexpr (E % R == 0) fft_stage<R, E, DIR>(v, t, T, sp, tw, lds, es, last);
+}
+
+// Full line FFT (unnormalized). DIR = -1 forward, +1 inverse. Must be called by every thread
+// of the workgroup (it contains barriers). On entry and exit v[e] holds element t + T*e.
+// lds: this line's LDS base; element n lives at lds[lds_pad(n) * es].
+template <int E, int DIR>
+__device__ __forceinline__ void fft_line(float2 (&v)[E], float2* __restrict__ lds, const int es,
+                                         const int t, const LinePlan& pl,
+                                         const float2* __restrict__ tw) {
+  const int T = pl.T;
+  for (int s = 0; s < pl.ns; ++s) {
+    const StagePlan sp = pl.st[s];
+    if (s > 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = lds[lds_pad(t + T * e) * es];
+    }
+    const bool last = (s + 1 == pl.ns);
+    switch (sp.R) {
+      case 2: fft_stage_if<2, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 3: fft_stage_if<3, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 4: fft_stage_if<4, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 5: fft_stage_if<5, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 8: fft_stage_if<8, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 16: fft_stage_if<16, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      default: break;
+    }
+  }
+}
+
+}  // namespace holo

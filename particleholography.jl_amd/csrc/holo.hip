@@ -1,0 +1,734 @@
+// libholo: host runtime and C ABI of the angular-spectrum / Gerchberg-Saxton path.
+//
+// The handle owns the FFT plans (radix schedule + FP64-rounded twiddle tables per line
+// length), a grow-only device workspace, a capture stream and a cache of instantiated
+// hipGraphs for the Gerchberg-Saxton iteration. Every entry point validates its arguments
+// on the host (the reference's @assert / ArgumentError cases become error codes), then
+// enqueues kernels on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "holo.h"
+#include "holo_kernels.hpp"
+
+using namespace holo;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HOLO_HIP(x)                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(HOLO_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HOLO_TRY(x)              \
+  do {                           \
+    int r_ = (x);                \
+    if (r_ != HOLO_OK) return r_; \
+  } while (0)
+
+struct Plan1D {
+  LinePlan lp;
+  int E = 0;
+  float2* dtw = nullptr;
+};
+
+struct Tuning {
+  int batch = 0, spg_cols = 0, spg_rows = 0, cols = 0, rows = 0;
+};
+
+// E = elements per thread; the set below is what the kernels are instantiated for.
+int choose_E(int L) {
+  if (L < 1 || L > 4096) return 0;
+  int o = L;
+  while ((o & 1) == 0) o >>= 1;
+  const int cap = o == 1 ? 16 : o == 3 ? 12 : o == 5 ? 20 : 0;
+  if (cap == 0) return 0;
+  return L <= cap ? L : cap;
+}
+
+template <class F>
+bool dispatch_E(int E, F&& f) {
+  switch (E) {
+#define HOLO_E(e) \
+  case e:         \
+    f(std::integral_constant<int, e>{}); \
+    return true;
+    HOLO_E(1) HOLO_E(2) HOLO_E(4) HOLO_E(8) HOLO_E(16)
+    HOLO_E(3) HOLO_E(6) HOLO_E(12)
+    HOLO_E(5) HOLO_E(10) HOLO_E(20)
+#undef HOLO_E
+    default: return false;
+  }
+}
+
+size_t env_size(const char* name, size_t dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  return static_cast<size_t>(std::strtoull(v, nullptr, 10));
+}
+
+}  // namespace
+
+struct holo_context {
+  int device = 0;
+  int max_lds = 65536;
+  std::map<int, Plan1D> plans;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  Tuning tune;
+  hipStream_t cap = nullptr;
+  double2* partials = nullptr;  // 256 partial sums for cu_2d_pad's mean
+  struct GsKey {
+    int nx, ny;
+    const void *buf, *i1, *i2, *hf, *hb;
+    double zf, zb, wl, dx;
+    int gf, gb, W, rows;
+    bool operator<(const GsKey& o) const {
+      return std::tie(nx, ny, buf, i1, i2, hf, hb, zf, zb, wl, dx, gf, gb, W, rows) <
+             std::tie(o.nx, o.ny, o.buf, o.i1, o.i2, o.hf, o.hb, o.zf, o.zb, o.wl, o.dx, o.gf, o.gb, o.W, o.rows);
+    }
+  };
+  std::map<GsKey, hipGraphExec_t> gs_graphs;
+};
+
+namespace {
+
+int get_plan(holo_context* h, int L, const Plan1D** out) {
+  auto it = h->plans.find(L);
+  if (it != h->plans.end()) {
+    *out = &it->second;
+    return HOLO_OK;
+  }
+  const int E = choose_E(L);
+  if (E == 0)
+    return fail(HOLO_ERR_SIZE, "FFT length " + std::to_string(L) +
+                                   " not supported (need 2^a, 3*2^a or 5*2^a, <= 4096)");
+  Plan1D p;
+  p.E = E;
+  p.lp.L = L;
+  p.lp.T = L / E;
+  p.lp.ns = 0;
+  static const int kRadices[] = {16, 8, 5, 4, 3, 2};
+  int rem = L, span = 1, twoff = 0;
+  std::vector<float2> tw;
+  while (rem > 1) {
+    int R = 0;
+    for (int r : kRadices)
+      if (E % r == 0 && rem % r == 0) {
+        R = r;
+        break;
+      }
+    if (R == 0 || p.lp.ns >= kMaxStages) return fail(HOLO_ERR_SIZE, "no radix plan for length " + std::to_string(L));
+    StagePlan& sp = p.lp.st[p.lp.ns++];
+    sp.R = R;
+    sp.p = span;
+    sp.pmagic = span >= 2 ? static_cast<uint32_t>((0x100000000ull + span - 1) / span) : 0u;
+    sp.twoff = twoff;
+    if (span > 1) {
+      const long long PR = static_cast<long long>(span) * R;
+      for (int m = 1; m < R; ++m)
+        for (int k = 0; k < span; ++k) {
+          const long long mk = (static_cast<long long>(m) * k) % PR;
+          const double a = -2.0 * M_PI * static_cast<double>(mk) / static_cast<double>(PR);
+          tw.push_back(make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a))));
+        }
+      twoff += (R - 1) * span;
+    }
+    span *= R;
+    rem /= R;
+  }
+  if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+  HOLO_HIP(hipMalloc(&p.dtw, tw.size() * sizeof(float2)));
+  HOLO_HIP(hipMemcpy(p.dtw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+  auto ins = h->plans.emplace(L, p);
+  *out = &ins.first->second;
+  return HOLO_OK;
+}
+
+int floor_pow2(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+int ilog2(int v) {
+  int s = 0;
+  while ((1 << s) < v) ++s;
+  return s;
+}
+
+size_t row_lds(const Geom& g) { return static_cast<size_t>(g.rows) * lds_padded_len(g.nx) * sizeof(float2); }
+size_t col_lds(const Geom& g) { return static_cast<size_t>(g.W) * lds_padded_len(g.ny) * sizeof(float2); }
+
+int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey) {
+  if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  const Plan1D *px, *py;
+  HOLO_TRY(get_plan(h, nx, &px));
+  HOLO_TRY(get_plan(h, ny, &py));
+  std::memset(g, 0, sizeof(Geom));
+  g->nx = nx;
+  g->ny = ny;
+  g->px = px->lp;
+  g->py = py->lp;
+  g->twx = px->dtw;
+  g->twy = py->dtw;
+  *Ex = px->E;
+  *Ey = py->E;
+  const size_t colbytes = static_cast<size_t>(lds_padded_len(ny)) * sizeof(float2);
+  int W = h->tune.cols;
+  if (W <= 0) {
+    W = 16;
+    while (W > 1 && (W * py->lp.T > 512 || W * colbytes > static_cast<size_t>(h->max_lds))) W /= 2;
+    W = std::min(W, floor_pow2(nx));
+  }
+  W = floor_pow2(std::max(W, 1));
+  if (W * py->lp.T > 512 || W * colbytes > static_cast<size_t>(h->max_lds))
+    return fail(HOLO_ERR_ARG, "column tile too large for a workgroup");
+  g->W = W;
+  g->wshift = ilog2(W);
+  const size_t rowbytes = static_cast<size_t>(lds_padded_len(nx)) * sizeof(float2);
+  int rows = h->tune.rows;
+  if (rows <= 0) rows = std::max(1, 256 / px->lp.T);
+  rows = std::min(rows, ny);
+  while (rows > 1 && (rows * px->lp.T > 512 || rows * rowbytes > static_cast<size_t>(h->max_lds))) --rows;
+  if (rows * px->lp.T > 512 || rows * rowbytes > static_cast<size_t>(h->max_lds))
+    return fail(HOLO_ERR_ARG, "row tile too large for a workgroup");
+  g->rows = rows;
+  return HOLO_OK;
+}
+
+dim3 row_grid(const Geom& g, int y = 1) { return dim3((g.ny + g.rows - 1) / g.rows, y); }
+dim3 row_block(const Geom& g) { return dim3(g.rows * g.px.T); }
+dim3 col_grid(const Geom& g, int y = 1) { return dim3((g.nx + g.W - 1) / g.W, y); }
+dim3 col_block(const Geom& g) { return dim3(g.W * g.py.T); }
+
+// Grow-only workspace. Growing preserves the old contents (callers may keep data at the
+// start of the workspace across a call that needs more room).
+int ensure_ws(holo_context* h, size_t bytes) {
+  if (bytes <= h->ws_bytes) return HOLO_OK;
+  void* nw = nullptr;
+  HOLO_HIP(hipDeviceSynchronize());
+  if (hipMalloc(&nw, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(HOLO_ERR_NOMEM, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  if (h->ws) {
+    HOLO_HIP(hipMemcpy(nw, h->ws, h->ws_bytes, hipMemcpyDeviceToDevice));
+    HOLO_HIP(hipFree(h->ws));
+  }
+  h->ws = nw;
+  h->ws_bytes = bytes;
+  return HOLO_OK;
+}
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(HOLO_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return HOLO_OK;
+}
+
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+}
+
+// ---- launch helpers ---------------------------------------------------------------------
+// rows: DIR -1 forward / +1 inverse; SRC 0 complex, 1 sqrt(real)
+int launch_rows_fft(const Geom& g, int Ex, int dir, int src, const void* in, size_t in_stride,
+                    float2* out, size_t out_stride, float scale, int nbatch, hipStream_t st) {
+  const size_t lds = row_lds(g);
+  bool ok = dispatch_E(Ex, [&](auto ec) {
+    constexpr int E = decltype(ec)::value;
+    auto launch = [&](auto kern) {
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, row_grid(g, nbatch), row_block(g), lds, st, in, in_stride, out, out_stride, scale, g);
+    };
+    if (dir < 0 && src == 0) launch(k_rows_fft<E, -1, 0>);
+    else if (dir < 0) launch(k_rows_fft<E, -1, 1>);
+    else launch(k_rows_fft<E, +1, 0>);
+  });
+  if (!ok) return fail(HOLO_ERR_SIZE, "no kernel for this row length");
+  return check_launch();
+}
+
+int launch_cols_fft(const Geom& g, int Ey, int dir, const float2* in, size_t in_stride, float2* out,
+                    size_t out_stride, float scale, int nbatch, hipStream_t st) {
+  const size_t lds = col_lds(g);
+  bool ok = dispatch_E(Ey, [&](auto ec) {
+    constexpr int E = decltype(ec)::value;
+    auto launch = [&](auto kern) {
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, col_grid(g, nbatch), col_block(g), lds, st, in, in_stride, out, out_stride, scale, g);
+    };
+    if (dir < 0) launch(k_cols_fft<E, -1>);
+    else launch(k_cols_fft<E, +1>);
+  });
+  if (!ok) return fail(HOLO_ERR_SIZE, "no kernel for this column length");
+  return check_launch();
+}
+
+int launch_fill(float* p, size_t n, unsigned int v, hipStream_t st) {
+  const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, st, reinterpret_cast<unsigned int*>(p), n, v);
+  return check_launch();
+}
+
+int validate_transfer(const holo_tf* t, const char* name) {
+  if (!t) return fail(HOLO_ERR_ARG, std::string(name) + " transfer is NULL");
+  if (t->generated) {
+    if (!(t->wavlen > 0) || !(t->dx > 0) || !std::isfinite(t->z))
+      return fail(HOLO_ERR_ARG, std::string(name) + ": generated transfer needs finite z and positive wavlen, dx");
+  } else if (!t->data) {
+    return fail(HOLO_ERR_ARG, std::string(name) + ": transfer array is NULL");
+  }
+  return HOLO_OK;
+}
+
+// Forward 2-D spectrum of the wavefront (complex) or of sqrt(holo) (real) into F.
+int forward_spectrum(const Geom& g, int Ex, int Ey, const void* src, int src_real, float2* F,
+                     hipStream_t st) {
+  HOLO_TRY(launch_rows_fft(g, Ex, -1, src_real, src, 0, F, 0, 1.0f, 1, st));
+  return launch_cols_fft(g, Ey, -1, F, 0, F, 0, 1.0f, 1, st);
+}
+
+struct ReconPlan {
+  int m;        // slices per batch
+  int spg_c;    // slices per column workgroup
+  int spg_r;    // slices per row workgroup
+};
+
+ReconPlan recon_plan(holo_context* h, const Geom& g, int slices) {
+  ReconPlan r;
+  const size_t plane = static_cast<size_t>(g.nx) * g.ny * sizeof(float2);
+  const size_t budget = env_size("HOLO_BATCH_BYTES", 128ull << 20);
+  int m = h->tune.batch > 0 ? h->tune.batch : static_cast<int>(std::max<size_t>(1, budget / plane));
+  r.m = std::max(1, std::min(m, slices));
+  r.spg_c = h->tune.spg_cols > 0 ? h->tune.spg_cols : 2;
+  r.spg_r = h->tune.spg_rows > 0 ? h->tune.spg_rows : 4;
+  return r;
+}
+
+// Core of A6-A9 and the padded variant. kind: 0 projection (fminf, A6); 1 store (A7-A9).
+// The workspace holds [reserve bytes | F | G (batch of slices) | running spectrum].
+int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src, int src_real,
+                const holo_tf* front, const holo_tf* dz, int slices, int kind,
+                int out_dtype, void* vol, float* proj, int ox, int oy, int onx, int ony,
+                size_t reserve = 0) {
+  HOLO_TRY(validate_transfer(front, "front"));
+  HOLO_TRY(validate_transfer(dz, "dz"));
+  if (front->generated != dz->generated)
+    return fail(HOLO_ERR_ARG, "front and dz transfers must both be generated or both be arrays");
+  if (front->generated && (front->wavlen != dz->wavlen || front->dx != dz->dx))
+    return fail(HOLO_ERR_ARG, "generated transfers must share wavlen and dx");
+  if (slices < 1) return fail(HOLO_ERR_ARG, "slices must be >= 1");
+  if (!src) return fail(HOLO_ERR_ARG, "input is NULL");
+  Geom g;
+  int Ex, Ey;
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  const ReconPlan rp = recon_plan(h, g, slices);
+  const size_t plane = static_cast<size_t>(nx) * ny;
+  const bool arrays = !front->generated;
+  const size_t need = reserve + plane * sizeof(float2) * (1 + static_cast<size_t>(rp.m) + (arrays ? 1 : 0));
+  HOLO_TRY(ensure_ws(h, need));
+  float2* F = reinterpret_cast<float2*>(static_cast<char*>(h->ws) + reserve);
+  float2* G = F + plane;
+  float2* run = G + plane * rp.m;
+  const size_t oplane = static_cast<size_t>(onx) * ony;
+  if (proj) HOLO_TRY(launch_fill(proj, oplane, 0x7fffffffu, st));
+  HOLO_TRY(forward_spectrum(g, Ex, Ey, src, src_real, F, st));
+  const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
+  const size_t clds = col_lds(g), rlds = row_lds(g);
+  for (int b0 = 0; b0 < slices; b0 += rp.m) {
+    const int nb = std::min(rp.m, slices - b0);
+    RecArgs ra{};
+    ra.F = F;
+    ra.G = G;
+    ra.gstride = plane;
+    ra.nbatch = nb;
+    ra.spg = rp.spg_c;
+    if (!arrays) {
+      ra.gp.wavlen = front->wavlen;
+      ra.gp.dx = front->dx;
+      ra.gp.zl0 = (front->z + static_cast<double>(b0) * dz->z) / front->wavlen;
+      ra.gp.dzl = dz->z / front->wavlen;
+    } else {
+      ra.H0 = static_cast<const float2*>(front->data);
+      ra.Hdz = static_cast<const float2*>(dz->data);
+      ra.run = run;
+      ra.first = b0 == 0;
+    }
+    bool ok = dispatch_E(Ey, [&](auto ec) {
+      constexpr int E = decltype(ec)::value;
+      if (!arrays) {
+        allow_lds(k_recon_cols<E, 0>, clds);
+        hipLaunchKernelGGL((k_recon_cols<E, 0>), col_grid(g, (nb + rp.spg_c - 1) / rp.spg_c), col_block(g), clds, st, ra, g);
+      } else {
+        allow_lds(k_recon_cols<E, 1>, clds);
+        hipLaunchKernelGGL((k_recon_cols<E, 1>), col_grid(g, 1), col_block(g), clds, st, ra, g);
+      }
+    });
+    if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
+    HOLO_TRY(check_launch());
+    const int ry = (nb + rp.spg_r - 1) / rp.spg_r;
+    ok = dispatch_E(Ex, [&](auto ec) {
+      constexpr int E = decltype(ec)::value;
+      if (kind == 0) {
+        allow_lds(k_recon_rows_proj<E>, rlds);
+        hipLaunchKernelGGL(k_recon_rows_proj<E>, row_grid(g, ry), row_block(g), rlds, st, G, plane, nb, rp.spg_r, proj, sc, g);
+      } else {
+        StoreArgs sa{};
+        sa.G = G;
+        sa.gstride = plane;
+        sa.nbatch = nb;
+        sa.spg = rp.spg_r;
+        sa.vol = vol;
+        sa.vstride = oplane;
+        sa.slice0 = b0;
+        sa.proj = proj;
+        sa.sc = sc;
+        sa.ox = ox;
+        sa.oy = oy;
+        sa.onx = onx;
+        sa.ony = ony;
+        auto launch = [&](auto kern) {
+          allow_lds(kern, rlds);
+          hipLaunchKernelGGL(kern, row_grid(g, ry), row_block(g), rlds, st, sa, g);
+        };
+        switch (out_dtype) {
+          case HOLO_N0F8: launch(k_recon_rows_store<E, 0>); break;
+          case HOLO_F16: launch(k_recon_rows_store<E, 1>); break;
+          case HOLO_F32: launch(k_recon_rows_store<E, 2>); break;
+          case HOLO_F64: launch(k_recon_rows_store<E, 3>); break;
+          default: launch(k_recon_rows_store<E, 4>); break;
+        }
+      }
+    });
+    if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
+    HOLO_TRY(check_launch());
+  }
+  return HOLO_OK;
+}
+
+// One propagation column pass: in -> FFT_col -> .* H -> IFFT_col -> out
+int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, float2* out,
+                     hipStream_t st) {
+  PropArgs pa{};
+  pa.in = in;
+  pa.out = out;
+  if (t->generated) {
+    pa.gp.wavlen = t->wavlen;
+    pa.gp.dx = t->dx;
+    pa.gp.zl0 = t->z / t->wavlen;
+  } else {
+    pa.H = static_cast<const float2*>(t->data);
+  }
+  const size_t lds = col_lds(g);
+  bool ok = dispatch_E(Ey, [&](auto ec) {
+    constexpr int E = decltype(ec)::value;
+    if (t->generated) {
+      allow_lds(k_prop_cols<E, 0>, lds);
+      hipLaunchKernelGGL((k_prop_cols<E, 0>), col_grid(g), col_block(g), lds, st, pa, g);
+    } else {
+      allow_lds(k_prop_cols<E, 1>, lds);
+      hipLaunchKernelGGL((k_prop_cols<E, 1>), col_grid(g), col_block(g), lds, st, pa, g);
+    }
+  });
+  if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
+  return check_launch();
+}
+
+int launch_gs_rows(const Geom& g, int Ex, int mode, const float2* U, const float* I, float2* out,
+                   float sc, hipStream_t st) {
+  const size_t lds = row_lds(g);
+  bool ok = dispatch_E(Ex, [&](auto ec) {
+    constexpr int E = decltype(ec)::value;
+    if (mode == 0) {
+      allow_lds(k_gs_rows<E, 0>, lds);
+      hipLaunchKernelGGL((k_gs_rows<E, 0>), row_grid(g), row_block(g), lds, st, U, I, out, sc, g);
+    } else {
+      allow_lds(k_gs_rows<E, 1>, lds);
+      hipLaunchKernelGGL((k_gs_rows<E, 1>), row_grid(g), row_block(g), lds, st, U, I, out, sc, g);
+    }
+  });
+  if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
+  return check_launch();
+}
+
+int gs_iteration(const Geom& g, int Ex, int Ey, float2* B, const float* I1, const float* I2,
+                 const holo_tf* fwd, const holo_tf* bwd, float sc, bool last,
+                 float2* out, hipStream_t st) {
+  HOLO_TRY(launch_prop_cols(g, Ey, fwd, B, B, st));
+  HOLO_TRY(launch_gs_rows(g, Ex, 0, B, I2, B, sc, st));
+  HOLO_TRY(launch_prop_cols(g, Ey, bwd, B, B, st));
+  return launch_gs_rows(g, Ex, last ? 1 : 0, B, I1, last ? out : B, sc, st);
+}
+
+}  // namespace
+
+// =========================================================================================
+// C ABI
+// =========================================================================================
+extern "C" {
+
+int holo_version(void) { return 1; }
+
+const char* holo_last_error(void) { return g_err.c_str(); }
+
+int holo_supported_length(int n) { return choose_E(n) != 0 ? 1 : 0; }
+
+int holo_create(holo_handle* out, int device) {
+  if (!out) return fail(HOLO_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  HOLO_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(HOLO_ERR_ARG, "device index out of range");
+  HOLO_HIP(hipSetDevice(device));
+  holo_context* h = new holo_context();
+  h->device = device;
+  int lds = 65536;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    h->max_lds = lds;
+  const size_t cap = env_size("HOLO_MAX_LDS", 0);
+  if (cap > 0) h->max_lds = static_cast<int>(std::min<size_t>(cap, h->max_lds));
+  if (hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(HOLO_ERR_HIP, "capture stream creation failed");
+  }
+  if (hipMalloc(&h->partials, 256 * sizeof(double2)) != hipSuccess) {
+    (void)hipStreamDestroy(h->cap);
+    delete h;
+    return fail(HOLO_ERR_NOMEM, "partial-sum buffer allocation failed");
+  }
+  *out = h;
+  return HOLO_OK;
+}
+
+int holo_destroy(holo_handle h) {
+  if (!h) return HOLO_OK;
+  (void)hipSetDevice(h->device);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : h->gs_graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : h->plans) (void)hipFree(kv.second.dtw);
+  if (h->ws) (void)hipFree(h->ws);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
+  if (h->partials) (void)hipFree(h->partials);
+  delete h;
+  return HOLO_OK;
+}
+
+int holo_set_tuning(holo_handle h, int batch, int spg_cols, int spg_rows, int cols_per_wg,
+                    int rows_per_wg) {
+  if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
+  if (cols_per_wg > 0 && (cols_per_wg & (cols_per_wg - 1)) != 0)
+    return fail(HOLO_ERR_ARG, "cols_per_wg must be a power of two");
+  h->tune.batch = batch;
+  h->tune.spg_cols = spg_cols;
+  h->tune.spg_rows = spg_rows;
+  h->tune.cols = cols_per_wg;
+  h->tune.rows = rows_per_wg;
+  for (auto& kv : h->gs_graphs) (void)hipGraphExecDestroy(kv.second);
+  h->gs_graphs.clear();
+  return HOLO_OK;
+}
+
+int holo_transfer_sqrt(holo_handle h, void* stream, int nx, int ny, double wavlen, double dx,
+                       int dtype, void* out) {
+  if (!h || !out) return fail(HOLO_ERR_ARG, "NULL handle or output");
+  if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid((nx + 255) / 256, ny), block(256);
+  if (dtype == HOLO_F32)
+    hipLaunchKernelGGL(k_transfer_sqrt<float>, grid, block, 0, st, nx, ny, wavlen, dx, static_cast<float*>(out));
+  else if (dtype == HOLO_F64)
+    hipLaunchKernelGGL(k_transfer_sqrt<double>, grid, block, 0, st, nx, ny, wavlen, dx, static_cast<double*>(out));
+  else
+    return fail(HOLO_ERR_TYPE, "transfer_sqrt dtype must be HOLO_F32 or HOLO_F64");
+  return check_launch();
+}
+
+int holo_transfer(holo_handle h, void* stream, int nx, int ny, double z, double wavlen,
+                  const void* d_sqr, int d_dtype, int precision, double dx, void* out) {
+  if (!h || !out) return fail(HOLO_ERR_ARG, "NULL handle or output");
+  if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  if (precision != HOLO_TF_FP64 && !d_sqr) return fail(HOLO_ERR_ARG, "d_sqr is NULL");
+  if (precision == HOLO_TF_FP64 && !(dx > 0)) return fail(HOLO_ERR_ARG, "dx must be positive");
+  if (precision < 0 || precision > 2) return fail(HOLO_ERR_ARG, "bad precision");
+  if (d_dtype != HOLO_F32 && d_dtype != HOLO_F64) return fail(HOLO_ERR_TYPE, "d_sqr dtype must be HOLO_F32 or HOLO_F64");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid((nx + 255) / 256, ny), block(256);
+  const double two_pi_z_over_l = (2.0 * M_PI * z) / wavlen;
+  const double z_over_l = z / wavlen;
+  if (d_dtype == HOLO_F32)
+    hipLaunchKernelGGL(k_transfer<float>, grid, block, 0, st, nx, ny, precision, two_pi_z_over_l, z_over_l,
+                       wavlen, dx, static_cast<const float*>(d_sqr), static_cast<float2*>(out));
+  else
+    hipLaunchKernelGGL(k_transfer<double>, grid, block, 0, st, nx, ny, precision, two_pi_z_over_l, z_over_l,
+                       wavlen, dx, static_cast<const double*>(d_sqr), static_cast<float2*>(out));
+  return check_launch();
+}
+
+int holo_gabor_wavefront(holo_handle h, void* stream, int nx, int ny, const float* holo, void* out) {
+  if (!h || !holo || !out) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  const size_t n = static_cast<size_t>(nx) * ny;
+  hipLaunchKernelGGL(k_gabor, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), n, holo, static_cast<float2*>(out));
+  return check_launch();
+}
+
+int holo_fft2(holo_handle h, void* stream, int nx, int ny, const void* in, void* out, int inverse) {
+  if (!h || !in || !out) return fail(HOLO_ERR_ARG, "NULL argument");
+  Geom g;
+  int Ex, Ey;
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int dir = inverse ? +1 : -1;
+  const float scale = inverse ? static_cast<float>(1.0 / (static_cast<double>(nx) * ny)) : 1.0f;
+  float2* o = static_cast<float2*>(out);
+  HOLO_TRY(launch_rows_fft(g, Ex, dir, 0, in, 0, o, 0, 1.0f, 1, st));
+  return launch_cols_fft(g, Ey, dir, o, 0, o, 0, scale, 1, st);
+}
+
+int holo_asm_prop(holo_handle h, void* stream, int nx, int ny, void* out, const void* in,
+                  const holo_tf* tf) {
+  if (!h || !in || !out) return fail(HOLO_ERR_ARG, "NULL argument");
+  HOLO_TRY(validate_transfer(tf, "tf"));
+  Geom g;
+  int Ex, Ey;
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  const size_t plane = static_cast<size_t>(nx) * ny;
+  HOLO_TRY(ensure_ws(h, plane * sizeof(float2)));
+  float2* B = static_cast<float2*>(h->ws);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
+  HOLO_TRY(launch_rows_fft(g, Ex, -1, 0, in, 0, B, 0, 1.0f, 1, st));
+  HOLO_TRY(launch_prop_cols(g, Ey, tf, B, B, st));
+  return launch_rows_fft(g, Ex, +1, 0, B, 0, static_cast<float2*>(out), 0, sc, 1, st);
+}
+
+int holo_reconst_xyprojection(holo_handle h, void* stream, int nx, int ny, const void* wavefront,
+                              const holo_tf* front, const holo_tf* dz, int slices,
+                              float* proj) {
+  if (!h || !proj) return fail(HOLO_ERR_ARG, "NULL handle or output");
+  return reconstruct(h, static_cast<hipStream_t>(stream), nx, ny, wavefront, 0, front, dz, slices, 0,
+                     HOLO_F32, nullptr, proj, 0, 0, nx, ny);
+}
+
+int holo_reconst_xyprojection_holo(holo_handle h, void* stream, int nx, int ny, const float* holo,
+                                   const holo_tf* front, const holo_tf* dz, int slices,
+                                   float* proj) {
+  if (!h || !proj) return fail(HOLO_ERR_ARG, "NULL handle or output");
+  return reconstruct(h, static_cast<hipStream_t>(stream), nx, ny, holo, 1, front, dz, slices, 0,
+                     HOLO_F32, nullptr, proj, 0, 0, nx, ny);
+}
+
+int holo_reconst_vol(holo_handle h, void* stream, int nx, int ny, const void* wavefront,
+                     const holo_tf* front, const holo_tf* dz, int slices, int out_dtype,
+                     void* vol, float* proj) {
+  if (!h || !vol) return fail(HOLO_ERR_ARG, "NULL handle or volume");
+  if (out_dtype < HOLO_N0F8 || out_dtype > HOLO_C64)
+    return fail(HOLO_ERR_TYPE, "return_type must be N0f8, Float16, Float32, Float64 or ComplexF32");
+  if (out_dtype == HOLO_C64 && proj) return fail(HOLO_ERR_ARG, "complex volume has no projection");
+  return reconstruct(h, static_cast<hipStream_t>(stream), nx, ny, wavefront, 0, front, dz, slices, 1,
+                     out_dtype, vol, proj, 0, 0, nx, ny);
+}
+
+int holo_2d_pad(holo_handle h, void* stream, int nx, int ny, const void* in, void* out) {
+  if (!h || !in || !out) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  constexpr int kParts = 256;
+  double2* part = h->partials;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const size_t n = static_cast<size_t>(nx) * ny;
+  hipLaunchKernelGGL(k_partial_sum, dim3(kParts), dim3(256), 0, st, static_cast<const float2*>(in), n, part);
+  HOLO_TRY(check_launch());
+  hipLaunchKernelGGL(k_pad_fill, dim3((2 * nx + 255) / 256, 2 * ny), dim3(256), 0, st,
+                     static_cast<const float2*>(in), nx, ny, part, kParts, static_cast<float2*>(out));
+  return check_launch();
+}
+
+int holo_reconst_vol_and_xyprojection_padded(holo_handle h, void* stream, int nx, int ny,
+                                             const void* wavefront, const holo_tf* front,
+                                             const holo_tf* dz, int slices, uint8_t* vol,
+                                             float* proj) {
+  if (!h || !wavefront || !vol || !proj) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (nx < 2 || ny < 2 || (nx & 1) || (ny & 1))
+    return fail(HOLO_ERR_ARG, "padded reconstruction needs even sizes");
+  // workspace: [padded 2nx x 2ny wavefront | reconstruction buffers]
+  const size_t pbytes = static_cast<size_t>(4) * nx * ny * sizeof(float2);
+  HOLO_TRY(ensure_ws(h, pbytes));
+  void* padded = h->ws;
+  HOLO_TRY(holo_2d_pad(h, stream, nx, ny, wavefront, padded));
+  // reconstruct() grows the workspace behind the padded wavefront (growth preserves it)
+  const int r = reconstruct(h, static_cast<hipStream_t>(stream), 2 * nx, 2 * ny, padded, 0, front, dz,
+                            slices, 1, HOLO_N0F8, vol, proj, nx / 2, ny / 2, nx, ny, pbytes);
+  return r;
+}
+
+int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const float* holo1,
+                         const float* holo2, const holo_tf* fwd, const holo_tf* bwd,
+                         int iters, void* out, int use_graph) {
+  if (!h || !holo1 || !holo2 || !out) return fail(HOLO_ERR_ARG, "NULL argument");
+  HOLO_TRY(validate_transfer(fwd, "transfer"));
+  HOLO_TRY(validate_transfer(bwd, "invtransfer"));
+  if (iters < 0) return fail(HOLO_ERR_ARG, "iterations must be >= 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (iters == 0) return holo_gabor_wavefront(h, stream, nx, ny, holo1, out);
+  Geom g;
+  int Ex, Ey;
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  const size_t plane = static_cast<size_t>(nx) * ny;
+  HOLO_TRY(ensure_ws(h, plane * sizeof(float2)));
+  float2* B = static_cast<float2*>(h->ws);
+  const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
+  float2* o = static_cast<float2*>(out);
+  HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, holo1, 0, B, 0, 1.0f, 1, st));  // psi1 = sqrt(I1), row FFT
+  const int body = iters - 1;
+  if (use_graph && body > 0) {
+    holo_context::GsKey key{nx, ny, B, holo1, holo2, fwd->data, bwd->data, fwd->z, bwd->z,
+                            fwd->wavlen, fwd->dx, fwd->generated, bwd->generated, g.W, g.rows};
+    hipGraphExec_t exec = nullptr;
+    auto it = h->gs_graphs.find(key);
+    if (it != h->gs_graphs.end()) {
+      exec = it->second;
+    } else {
+      hipGraph_t graph = nullptr;
+      HOLO_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
+      const int r = gs_iteration(g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, false, o, h->cap);
+      const hipError_t e = hipStreamEndCapture(h->cap, &graph);
+      if (r != HOLO_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+      }
+      HOLO_HIP(e);
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HOLO_HIP(ei);
+      h->gs_graphs.emplace(key, exec);
+    }
+    for (int it2 = 0; it2 < body; ++it2) HOLO_HIP(hipGraphLaunch(exec, st));
+  } else {
+    for (int it2 = 0; it2 < body; ++it2)
+      HOLO_TRY(gs_iteration(g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, false, o, st));
+  }
+  return gs_iteration(g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, true, o, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,558 @@
+// HIP kernels of the angular-spectrum / Gerchberg-Saxton path (gfx950).
+//
+// Memory layout: an nx x ny plane is stored with the x axis contiguous (the reference's Julia
+// column-major CuArray layout, dim 1 = x): element (i, j) at j*nx + i. "Row" passes run the
+// FFT along x (contiguous lines), "column" passes along y (stride nx; a workgroup owns W
+// adjacent columns so that every global access is a W*8-byte contiguous segment).
+//
+// Spectra are kept in natural FFT order. The reference multiplies fftshift(fft(psi)) by a
+// centred transfer array H and applies ifftshift before the inverse FFT; since
+// ifftshift(fftshift(X) .* H) == X .* ifftshift(H) for every size, the kernels either
+// generate ifftshift(H) directly or read the centred array at the shifted index.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "fft_core.hpp"
+
+namespace holo {
+
+struct Geom {
+  int nx, ny;
+  LinePlan px;  // plan along x (row passes)
+  LinePlan py;  // plan along y (column passes)
+  const float2* twx;
+  const float2* twy;
+  int W;        // columns per workgroup in column passes (power of two)
+  int wshift;   // log2(W)
+  int rows;     // rows per workgroup in row passes
+};
+
+// Parameters of the FP64 transfer-function generator (natural FFT order).
+//   a(n, N, c) = (((ic - N/2) * wavlen) / N / dx)^2 with ic = (n + N/2) mod N  (holofunc.jl:12)
+//   d = (1 - a_x) - a_y;  H = exp(2 pi i frac(z/wavlen * sqrt(d)))
+struct GenParams {
+  double wavlen, dx;
+  double zl0;   // z / wavlen of the first slice handled by the launch (cycles per unit sqrt)
+  double dzl;   // dz / wavlen between slices
+};
+
+__device__ __forceinline__ double axis_a(int n, int N, double wavlen, double dx) {
+#pragma clang fp contract(off)
+  const int ic = (n + (N >> 1)) % N;  // natural index -> centred index
+  const double t = (static_cast<double>(ic) - 0.5 * N) * wavlen / N / dx;
+  return t * t;
+}
+
+// exp(2 pi i u) for u in cycles, with u wrapped to [-1/2, 1/2] in FP64 before the FP32
+// sincospi. Evanescent components (d < 0) are returned as 0.
+__device__ __forceinline__ float2 cis_cycles(double u) {
+  const double f = u - rint(u);
+  float s, c;
+  sincospif(static_cast<float>(2.0 * f), &s, &c);
+  return make_float2(c, s);
+}
+
+// ------------------------------------------------------------------------------------------
+// A2 / A3 / A4: elementwise constructors (centred layout, the reference's arrays)
+// ------------------------------------------------------------------------------------------
+// d[x,y] = 1 - ax - ay in FP64 with the reference's op order, stored as F32 or F64
+// (holofunc.jl:8-15).
+template <typename T>
+__global__ void k_transfer_sqrt(int nx, int ny, double wavlen, double dx, T* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  const double tx = (static_cast<double>(i) - 0.5 * nx) * wavlen / nx / dx;
+  const double ty = (static_cast<double>(j) - 0.5 * ny) * wavlen / ny / dx;
+  const double d = (1.0 - tx * tx) - ty * ty;
+  out[static_cast<size_t>(j) * nx + i] = static_cast<T>(d);
+}
+
+// H = exp(i * ((2 pi z)/wavlen) * s) rounded to ComplexF32 (holofunc.jl:38-45).
+//   mode 0 (reference): s = Float32 sqrt of the Float32 d array, FP64 sincos of the full
+//                       angle (the reference's arithmetic, including its FP32 sqrt).
+//   mode 1: s = FP64 sqrt of the given d values (F32 or F64), phase wrapped in FP64.
+//   mode 2: s = FP64 sqrt of d recomputed in FP64 from (wavlen, dx), phase wrapped in FP64.
+template <typename DT>
+__global__ void k_transfer(int nx, int ny, int mode, double two_pi_z_over_l, double z_over_l,
+                           double wavlen, double dx, const DT* __restrict__ dsq,
+                           float2* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  const size_t idx = static_cast<size_t>(j) * nx + i;
+  float2 h;
+  if (mode == 0) {
+    const float d32 = static_cast<float>(dsq[idx]);
+    const float s32 = sqrtf(d32);  // NaN for d < 0, as in the reference
+    const double th = two_pi_z_over_l * static_cast<double>(s32);
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    h = make_float2(static_cast<float>(cs), static_cast<float>(sn));
+  } else {
+    double d;
+    if (mode == 1) {
+      d = static_cast<double>(dsq[idx]);
+    } else {
+      const double tx = (static_cast<double>(i) - 0.5 * nx) * wavlen / nx / dx;
+      const double ty = (static_cast<double>(j) - 0.5 * ny) * wavlen / ny / dx;
+      d = (1.0 - tx * tx) - ty * ty;
+    }
+    h = d >= 0.0 ? cis_cycles(z_over_l * sqrt(d)) : make_float2(0.f, 0.f);
+  }
+  out[idx] = h;
+}
+
+// psi = ComplexF32(sqrt(holo) + 0im)  (holofunc.jl:80-82)
+__global__ void k_gabor(size_t n, const float* __restrict__ holo, float2* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2(sqrtf(holo[i]), 0.f);
+}
+
+// ------------------------------------------------------------------------------------------
+// helpers shared by the pass kernels
+// ------------------------------------------------------------------------------------------
+struct RowCtx {
+  int t, w, j;
+  bool ok;
+  float2* line;
+};
+__device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
+  RowCtx c;
+  const int T = g.px.T;
+  c.t = threadIdx.x % T;
+  c.w = threadIdx.x / T;
+  c.j = blockIdx.x * g.rows + c.w;
+  c.ok = c.j < g.ny;
+  c.line = lds + c.w * lds_padded_len(g.nx);
+  return c;
+}
+
+struct ColCtx {
+  int t, w, i;
+  float2* col;
+};
+__device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
+  ColCtx c;
+  c.w = threadIdx.x & (g.W - 1);
+  c.t = threadIdx.x >> g.wshift;
+  c.i = blockIdx.x * g.W + c.w;
+  c.col = lds + c.w;
+  return c;
+}
+
+// ------------------------------------------------------------------------------------------
+// generic 2-D FFT passes (A12; forward spectrum of A5)
+// ------------------------------------------------------------------------------------------
+// Row pass with an optional scale on the output. SRC 0: complex input; SRC 1: real
+// intensity I, input sqrt(I) (the Gabor wavefront, holofunc.jl:81, fused into the first FFT
+// pass).
+template <int E, int DIR, int SRC>
+__global__ __launch_bounds__(512) void k_rows_fft(const void* __restrict__ in, size_t in_stride,
+                                                  float2* __restrict__ out, size_t out_stride,
+                                                  float scale, Geom g) {
+  extern __shared__ float2 lds[];
+  const RowCtx c = row_ctx(g, lds);
+  const int T = g.px.T;
+  const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
+  const size_t zin = blockIdx.y * in_stride, zout = blockIdx.y * out_stride;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = c.t + T * e;
+    if (!c.ok) {
+      v[e] = make_float2(0.f, 0.f);
+    } else if (SRC == 0) {
+      v[e] = static_cast<const float2*>(in)[zin + rowoff + i];
+    } else {
+      v[e] = make_float2(sqrtf(static_cast<const float*>(in)[zin + rowoff + i]), 0.f);
+    }
+  }
+  fft_line<E, DIR>(v, c.line, 1, c.t, g.px, g.twx);
+  if (c.ok) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) out[zout + rowoff + c.t + T * e] = cscale(v[e], scale);
+  }
+}
+
+// Column pass with an optional scale on the output.
+template <int E, int DIR>
+__global__ __launch_bounds__(512) void k_cols_fft(const float2* __restrict__ in, size_t in_stride,
+                                                  float2* __restrict__ out, size_t out_stride,
+                                                  float scale, Geom g) {
+  extern __shared__ float2 lds[];
+  const ColCtx c = col_ctx(g, lds);
+  const int T = g.py.T;
+  const bool ok = c.i < g.nx;
+  const size_t zin = blockIdx.y * in_stride, zout = blockIdx.y * out_stride;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    v[e] = ok ? in[zin + static_cast<size_t>(c.t + T * e) * g.nx + c.i] : make_float2(0.f, 0.f);
+  fft_line<E, DIR>(v, c.col, g.W, c.t, g.py, g.twy);
+  if (ok) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      out[zout + static_cast<size_t>(c.t + T * e) * g.nx + c.i] = cscale(v[e], scale);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Reconstruction pass A (column inverse FFT of F .* H_k), A6-A9
+// ------------------------------------------------------------------------------------------
+// HMODE 0 (generated): H_k = exp(2 pi i frac((zl0 + k*dzl) * sqrt(d))) in FP64-wrapped form.
+//   The WG keeps F and sqrt(d) (FP64) for its W x ny tile in registers across its slices.
+// HMODE 1 (arrays, the reference's running product, holofunc.jl:254-260): the ComplexF32
+//   running spectrum R_k = R_{k-1} .* Hdz (R_0 = F .* H0) is advanced in registers; the WG
+//   covers every slice of the batch in order and stores R back to `run` at the end.
+// G receives slice k of the batch at G + k*gstride.
+struct RecArgs {
+  const float2* F;       // natural-order spectrum of the wavefront
+  float2* G;             // batch of column-transformed slices
+  size_t gstride;        // elements between slices of G
+  int nbatch;            // slices in this batch
+  int spg;               // slices per workgroup (HMODE 0)
+  GenParams gp;          // HMODE 0
+  const float2* H0;      // HMODE 1: centred front transfer
+  const float2* Hdz;     // HMODE 1: centred slice-step transfer
+  float2* run;           // HMODE 1: running spectrum (natural order)
+  int first;             // HMODE 1: this batch starts at slice 0 (R_0 = F .* H0)
+};
+
+__device__ __forceinline__ size_t centred_index(int i, int j, int nx, int ny) {
+  const int ic = (i + (nx >> 1)) % nx;
+  const int jc = (j + (ny >> 1)) % ny;
+  return static_cast<size_t>(jc) * nx + ic;
+}
+
+template <int E, int HMODE>
+__global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, Geom g) {
+  extern __shared__ float2 lds[];
+  const ColCtx c = col_ctx(g, lds);
+  const int T = g.py.T;
+  const bool ok = c.i < g.nx;
+  const int ci = ok ? c.i : 0;
+  float2 f[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) f[e] = ok ? a.F[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
+
+  if constexpr (HMODE == 0) {
+    double s[E];
+    {
+      const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+#pragma clang fp contract(off)
+        const double d = (1.0 - ax) - axis_a(c.t + T * e, g.ny, a.gp.wavlen, a.gp.dx);
+        s[e] = d >= 0.0 ? sqrt(d) : -1.0;  // -1 marks an evanescent component
+      }
+    }
+    const int k0 = blockIdx.y * a.spg;
+    const int k1 = min(k0 + a.spg, a.nbatch);
+    for (int k = k0; k < k1; ++k) {
+      const double zl = a.gp.zl0 + k * a.gp.dzl;
+      float2 v[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float2 h = s[e] >= 0.0 ? cis_cycles(zl * s[e]) : make_float2(0.f, 0.f);
+        v[e] = cmul(f[e], h);
+      }
+      fft_line<E, +1>(v, c.col, g.W, c.t, g.py, g.twy);
+      if (ok) {
+        float2* __restrict__ dst = a.G + static_cast<size_t>(k) * a.gstride;
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
+      }
+    }
+  } else {
+    float2 hdz[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = c.t + T * e;
+      hdz[e] = a.Hdz[centred_index(ci, j, g.nx, g.ny)];
+      if (a.first) {
+        f[e] = cmul(f[e], a.H0[centred_index(ci, j, g.nx, g.ny)]);
+      } else {
+        f[e] = ok ? a.run[static_cast<size_t>(j) * g.nx + ci] : make_float2(0.f, 0.f);
+      }
+    }
+    for (int k = 0; k < a.nbatch; ++k) {
+      if (k > 0 || !a.first) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) f[e] = cmul(f[e], hdz[e]);
+      }
+      float2 v[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = f[e];
+      fft_line<E, +1>(v, c.col, g.W, c.t, g.py, g.twy);
+      if (ok) {
+        float2* __restrict__ dst = a.G + static_cast<size_t>(k) * a.gstride;
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
+      }
+    }
+    if (ok) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) a.run[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = f[e];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Reconstruction pass B (row inverse FFT + epilogue)
+// ------------------------------------------------------------------------------------------
+// Pass B workgroups split the slices of a batch among themselves (blockIdx.y = slice group of
+// `spg` slices) and merge their minima with an atomic min on the bit pattern. Intensities are
+// >= +0, for which the IEEE bit pattern orders like an unsigned integer; the projection is
+// initialised to 0x7fffffff (a NaN that compares above every other non-negative pattern), so
+// the merged result equals fminf over all slices: a NaN is ignored unless every slice is NaN.
+__device__ __forceinline__ void atomic_min_nonneg(float* p, float v) {
+  atomicMin(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
+}
+
+__global__ void k_fill_u32(unsigned int* __restrict__ p, size_t n, unsigned int v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Projection epilogue (A6): proj = min over slices of |psi_k|^2, psi_k = IFFT/(nx*ny). The
+// minimum follows CUDA.jl's device min for Float32 (fminf: a NaN operand is ignored).
+template <int E>
+__global__ __launch_bounds__(512) void k_recon_rows_proj(const float2* __restrict__ G,
+                                                         size_t gstride, int nbatch, int spg,
+                                                         float* __restrict__ proj, float sc,
+                                                         Geom g) {
+  extern __shared__ float2 lds[];
+  const RowCtx c = row_ctx(g, lds);
+  const int T = g.px.T;
+  const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
+  const int k0 = blockIdx.y * spg;
+  const int k1 = min(k0 + spg, nbatch);
+  float mn[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) mn[e] = __uint_as_float(0x7fffffffu);
+  for (int k = k0; k < k1; ++k) {
+    float2 v[E];
+    const float2* __restrict__ src = G + static_cast<size_t>(k) * gstride + rowoff;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = c.ok ? src[c.t + T * e] : make_float2(0.f, 0.f);
+    fft_line<E, +1>(v, c.line, 1, c.t, g.px, g.twx);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float a = v[e].x * sc, b = v[e].y * sc;
+      mn[e] = fminf(mn[e], a * a + b * b);
+    }
+  }
+  if (c.ok && k1 > k0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) atomic_min_nonneg(proj + rowoff + c.t + T * e, mn[e]);
+  }
+}
+
+// Store epilogue (A7 volume, A8 complex volume, A9 projection of the stored values, and
+// the padded variant's centre window). OUT: 0 N0f8, 1 Float16, 2 Float32, 3 Float64,
+// 4 ComplexF32. Intensities are clamp(|psi|^2, 0, 1) (holofunc.jl:158-166); N0f8 is
+// round(UInt8, v*255) with ties to even, computed in FP64 like the reference's promotion.
+// A NaN intensity is stored as 0 in N0f8 (the reference raises InexactError there).
+struct StoreArgs {
+  const float2* G;
+  size_t gstride;
+  int nbatch;
+  int spg;             // slices per workgroup
+  void* vol;           // output volume, slice k of this batch at vol + (slice0 + k) * vstride
+  size_t vstride;      // elements per output slice
+  int slice0;
+  float* proj;         // optional: min over slices of the stored value (A9)
+  float sc;            // 1 / (nx * ny)
+  int ox, oy, onx, ony;  // output window (crop) in the transform's pixel grid
+};
+
+template <int OUT>
+__device__ __forceinline__ float store_value(void* vol, size_t idx, float2 psi) {
+  if constexpr (OUT == 4) {
+    static_cast<float2*>(vol)[idx] = psi;
+    return 0.f;
+  } else {
+    const float in = psi.x * psi.x + psi.y * psi.y;
+    const float cl = in != in ? in : fminf(fmaxf(in, 0.f), 1.f);
+    if constexpr (OUT == 0) {
+      const double r = rint(static_cast<double>(cl) * 255.0);
+      const unsigned char u = cl != cl ? 0 : static_cast<unsigned char>(r);
+      static_cast<unsigned char*>(vol)[idx] = u;
+      return static_cast<float>(u) / 255.0f;
+    } else if constexpr (OUT == 1) {
+      const __half h = __float2half_rn(cl);
+      static_cast<__half*>(vol)[idx] = h;
+      return __half2float(h);
+    } else if constexpr (OUT == 2) {
+      static_cast<float*>(vol)[idx] = cl;
+      return cl;
+    } else {
+      static_cast<double*>(vol)[idx] = static_cast<double>(cl);
+      return cl;
+    }
+  }
+}
+
+template <int E, int OUT>
+__global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
+  extern __shared__ float2 lds[];
+  const RowCtx c = row_ctx(g, lds);
+  const int T = g.px.T;
+  const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
+  const int oj = c.j - a.oy;
+  const bool rowin = c.ok && oj >= 0 && oj < a.ony;
+  const int k0 = blockIdx.y * a.spg;
+  const int k1 = min(k0 + a.spg, a.nbatch);
+  float mn[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) mn[e] = __uint_as_float(0x7fffffffu);
+  for (int k = k0; k < k1; ++k) {
+    float2 v[E];
+    const float2* __restrict__ src = a.G + static_cast<size_t>(k) * a.gstride + rowoff;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = c.ok ? src[c.t + T * e] : make_float2(0.f, 0.f);
+    fft_line<E, +1>(v, c.line, 1, c.t, g.px, g.twx);
+    if (rowin) {
+      const size_t sbase = static_cast<size_t>(a.slice0 + k) * a.vstride + static_cast<size_t>(oj) * a.onx;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int oi = c.t + T * e - a.ox;
+        if (oi >= 0 && oi < a.onx) mn[e] = fminf(mn[e], store_value<OUT>(a.vol, sbase + oi, cscale(v[e], a.sc)));
+      }
+    }
+  }
+  if (OUT != 4 && a.proj != nullptr && rowin && k1 > k0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int oi = c.t + T * e - a.ox;
+      if (oi >= 0 && oi < a.onx) atomic_min_nonneg(a.proj + static_cast<size_t>(oj) * a.onx + oi, mn[e]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Propagation column pass (GS, cu_asm_prop!): forward column FFT, .* H, inverse column FFT.
+// HMODE 0: H generated (FP64-wrapped, natural order); HMODE 1: centred ComplexF32 array.
+// ------------------------------------------------------------------------------------------
+struct PropArgs {
+  const float2* in;
+  float2* out;
+  GenParams gp;        // HMODE 0: zl0 = z / wavlen
+  const float2* H;     // HMODE 1
+};
+
+template <int E, int HMODE>
+__global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, Geom g) {
+  extern __shared__ float2 lds[];
+  const ColCtx c = col_ctx(g, lds);
+  const int T = g.py.T;
+  const bool ok = c.i < g.nx;
+  const int ci = ok ? c.i : 0;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    v[e] = ok ? a.in[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
+  fft_line<E, -1>(v, c.col, g.W, c.t, g.py, g.twy);
+  if constexpr (HMODE == 0) {
+    const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+#pragma clang fp contract(off)
+      const double d = (1.0 - ax) - axis_a(c.t + T * e, g.ny, a.gp.wavlen, a.gp.dx);
+      const float2 h = d >= 0.0 ? cis_cycles(a.gp.zl0 * sqrt(d)) : make_float2(0.f, 0.f);
+      v[e] = cmul(v[e], h);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cmul(v[e], a.H[centred_index(ci, c.t + T * e, g.nx, g.ny)]);
+  }
+  fft_line<E, +1>(v, c.col, g.W, c.t, g.py, g.twy);
+  if (ok) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) a.out[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
+  }
+}
+
+// GS row pass (holofunc.jl:115-125): inverse row FFT (scaled 1/(nx*ny)), amplitude
+// replacement psi <- sqrt(I) * psi/|psi| (|psi| = 0 -> sqrt(I), i.e. angle(0) = 0), then
+// MODE 0: forward row FFT of the result into `out` (first half of the next propagation);
+// MODE 1: store the result (the retrieved wavefront).
+template <int E, int MODE>
+__global__ __launch_bounds__(512) void k_gs_rows(const float2* __restrict__ U,
+                                                 const float* __restrict__ I,
+                                                 float2* __restrict__ out, float sc, Geom g) {
+  extern __shared__ float2 lds[];
+  const RowCtx c = row_ctx(g, lds);
+  const int T = g.px.T;
+  const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = c.ok ? U[rowoff + c.t + T * e] : make_float2(0.f, 0.f);
+  fft_line<E, +1>(v, c.line, 1, c.t, g.px, g.twx);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const float A = c.ok ? sqrtf(I[rowoff + c.t + T * e]) : 0.f;
+    const float a = v[e].x * sc, b = v[e].y * sc;
+    const float r2 = a * a + b * b;
+    if (r2 > 0.f) {
+      const float f = A / sqrtf(r2);
+      v[e] = make_float2(a * f, b * f);
+    } else {
+      v[e] = make_float2(A, 0.f);
+    }
+  }
+  if constexpr (MODE == 0) fft_line<E, -1>(v, c.line, 1, c.t, g.px, g.twx);
+  if (c.ok) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) out[rowoff + c.t + T * e] = v[e];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// f2: cu_2d_pad (holofunc.jl:139-149): mean-filled 2N x 2N with the input at offset N/2.
+// ------------------------------------------------------------------------------------------
+__global__ void k_partial_sum(const float2* __restrict__ in, size_t n, double2* __restrict__ part) {
+  __shared__ double sx[256], sy[256];
+  double ax = 0.0, ay = 0.0;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    ax += in[i].x;
+    ay += in[i].y;
+  }
+  sx[threadIdx.x] = ax;
+  sy[threadIdx.x] = ay;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sx[threadIdx.x] += sx[threadIdx.x + s];
+      sy[threadIdx.x] += sy[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sx[0], sy[0]);
+}
+
+__global__ void k_pad_fill(const float2* __restrict__ in, int nx, int ny,
+                           const double2* __restrict__ part, int nparts, float2* __restrict__ out) {
+  double mx = 0.0, my = 0.0;
+  for (int p = 0; p < nparts; ++p) {
+    mx += part[p].x;
+    my += part[p].y;
+  }
+  const double inv = 1.0 / (static_cast<double>(nx) * ny);
+  const float2 mean = make_float2(static_cast<float>(mx * inv), static_cast<float>(my * inv));
+  const int onx = 2 * nx, ony = 2 * ny;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= onx) return;
+  const int ox = onx / 4, oy = ony / 4;
+  const int si = i - ox, sj = j - oy;
+  const bool inside = si >= 0 && si < nx && sj >= 0 && sj < ny;
+  out[static_cast<size_t>(j) * onx + i] = inside ? in[static_cast<size_t>(sj) * nx + si] : mean;
+}
+
+}  // namespace holo

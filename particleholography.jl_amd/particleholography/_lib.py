@@ -1,0 +1,130 @@
+"""ctypes binding of libholo.so (the C ABI declared in include/holo.h).
+
+There is no fallback: if the library is missing or cannot be loaded, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from pathlib import Path
+
+PKG_ROOT = Path(__file__).resolve().parent.parent          # particleholography.jl_amd/
+LIB_PATH = Path(os.environ.get("HOLO_LIB", PKG_ROOT / "lib" / "libholo.so"))
+
+HOLO_OK, HOLO_ERR_ARG, HOLO_ERR_SIZE, HOLO_ERR_TYPE, HOLO_ERR_HIP, HOLO_ERR_NOMEM, HOLO_ERR_BUDGET = range(7)
+HOLO_N0F8, HOLO_F16, HOLO_F32, HOLO_F64, HOLO_C64 = range(5)
+HOLO_TF_REFERENCE, HOLO_TF_FP64_D, HOLO_TF_FP64 = range(3)
+
+
+class ArgumentError(ValueError):
+    """The reference's ArgumentError (e.g. an unsupported return_type, holofunc.jl:164)."""
+
+
+class HoloError(RuntimeError):
+    """HIP runtime / allocation failure inside libholo."""
+
+
+class holo_tf(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("z", ctypes.c_double), ("wavlen", ctypes.c_double),
+                ("dx", ctypes.c_double), ("generated", ctypes.c_int)]
+
+
+_vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+_tfp = ctypes.POINTER(holo_tf)
+
+# name -> argtypes (restype is int for all but holo_last_error)
+SIGNATURES = {
+    "holo_create": [ctypes.POINTER(_vp), _i],
+    "holo_destroy": [_vp],
+    "holo_last_error": [],
+    "holo_version": [],
+    "holo_supported_length": [_i],
+    "holo_set_tuning": [_vp, _i, _i, _i, _i, _i],
+    "holo_transfer_sqrt": [_vp, _vp, _i, _i, _d, _d, _i, _vp],
+    "holo_transfer": [_vp, _vp, _i, _i, _d, _d, _vp, _i, _i, _d, _vp],
+    "holo_gabor_wavefront": [_vp, _vp, _i, _i, _vp, _vp],
+    "holo_reconst_xyprojection": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _vp],
+    "holo_reconst_xyprojection_holo": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _vp],
+    "holo_reconst_vol": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _i, _vp, _vp],
+    "holo_2d_pad": [_vp, _vp, _i, _i, _vp, _vp],
+    "holo_reconst_vol_and_xyprojection_padded": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _vp, _vp],
+    "holo_phase_retrieval": [_vp, _vp, _i, _i, _vp, _vp, _tfp, _tfp, _i, _vp, _i],
+    "holo_asm_prop": [_vp, _vp, _i, _i, _vp, _vp, _tfp],
+    "holo_fft2": [_vp, _vp, _i, _i, _vp, _vp, _i],
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load() -> ctypes.CDLL:
+    """Load libholo.so once; raise if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise ImportError(f"libholo.so not found at {LIB_PATH}; build it with "
+                              f"`make -C {PKG_ROOT / 'csrc'}` (or __graft_entry__.build())")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "holo_last_error" else ctypes.c_int
+        _lib = lib
+        return lib
+
+
+def check(code: int) -> None:
+    if code == HOLO_OK:
+        return
+    msg = load().holo_last_error().decode(errors="replace")
+    if code == HOLO_ERR_ARG:
+        raise AssertionError(msg)
+    if code == HOLO_ERR_TYPE:
+        raise ArgumentError(msg)
+    if code == HOLO_ERR_SIZE:
+        raise ValueError(msg)
+    if code == HOLO_ERR_BUDGET:
+        raise MemoryError(msg)
+    raise HoloError(f"libholo error {code}: {msg}")
+
+
+class Handle:
+    """One libholo context (plans, workspace, graph cache) on one device."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = ctypes.c_void_p()
+        check(lib.holo_create(ctypes.byref(h), device))
+        self.ptr = h
+        self.device = device
+        self.lib = lib
+
+    def close(self):
+        if self.ptr:
+            self.lib.holo_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order varies
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, batch=0, spg_cols=0, spg_rows=0, cols_per_wg=0, rows_per_wg=0):
+        check(self.lib.holo_set_tuning(self.ptr, batch, spg_cols, spg_rows, cols_per_wg, rows_per_wg))
+
+
+_handles: dict = {}
+
+
+def handle(device: int = 0) -> Handle:
+    """Per (thread, device) handle: a libholo handle is not thread safe."""
+    key = (threading.get_ident(), device)
+    h = _handles.get(key)
+    if h is None:
+        h = Handle(device)
+        _handles[key] = h
+    return h

@@ -1,0 +1,253 @@
+"""Pins the CPU oracle (oracle/asm.py) with checks that do not restate its formulas:
+scalar brute force, closed forms of the propagation, invariants and the reference's own
+known-answer test. CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import asm
+
+LAM, DX = 0.6328, 10.0
+
+
+# ---------------------------------------------------------------------------------------
+# A2: d array, bit-exact against a scalar re-evaluation with Python floats
+# ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [4, 5, 8, 16, 33])
+def test_transfer_sqrt_scalar_bruteforce(n):
+    d = asm.transfer_sqrt_arr(n, LAM, DX)
+    for y in range(1, n + 1):          # Julia 1-based indices, holofunc.jl:12
+        for x in range(1, n + 1):
+            v = 1.0 - ((x - n / 2 - 1.0) * LAM / n / DX) ** 2 - ((y - n / 2 - 1.0) * LAM / n / DX) ** 2
+            assert d[y - 1, x - 1] == np.float32(v)
+
+
+def test_transfer_sqrt_centre_and_symmetry():
+    n = 64
+    d = asm.transfer_sqrt_arr(n, LAM, DX, dtype=np.float64)
+    assert d[n // 2, n // 2] == 1.0                       # x = n/2 + 1 is the DC term
+    assert np.all(d[n // 2, 1:] == d[n // 2, 1:][::-1])    # symmetric about the centre
+    fx = (np.arange(n) - n / 2) / (n * DX)
+    np.testing.assert_allclose(d[n // 2], 1.0 - (LAM * fx) ** 2, rtol=0, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------------------
+# A3: transfer functions
+# ---------------------------------------------------------------------------------------
+def test_transfer_reference_properties():
+    n = 64
+    d32 = asm.transfer_sqrt_arr(n, LAM, DX)
+    h0 = asm.transfer_reference(0.0, LAM, d32)
+    assert np.all(h0 == np.complex64(1.0))
+    h = asm.transfer_reference(-8e4, LAM, d32)
+    np.testing.assert_allclose(np.abs(h), 1.0, atol=2e-7)
+    # DC entry: sqrt(d) = 1 exactly, so H = exp(2 pi i z / lambda)
+    want = np.exp(2j * np.pi * (-8e4) / LAM)
+    assert abs(h[n // 2, n // 2] - want) < 2e-7
+
+
+def test_transfer_reference_phase_error_matches_float32_sqrt_bound():
+    # The reference rounds d and sqrt(d) to Float32 (holofunc.jl:12,42); the phase error vs
+    # the exact transfer grows linearly in z, bounded by |2 pi z / lambda| * (ulp terms).
+    n = 256
+    d32 = asm.transfer_sqrt_arr(n, LAM, DX)
+    for z in (2e3, 8e4):
+        href = asm.transfer_reference(z, LAM, d32)
+        hex_ = asm.transfer_exact(z, LAM, DX, n)
+        dphi = np.angle(href * np.conj(hex_))
+        bound = 2 * np.pi * abs(z) / LAM * 2 * np.finfo(np.float32).eps + 1e-6
+        assert np.max(np.abs(dphi)) <= bound
+    # and it is not zero at large z: the emulation really carries the F32 defect
+    assert np.max(np.abs(dphi)) > 1e-3
+
+
+def test_transfer_exact_composition():
+    n = 32
+    a = asm.transfer_exact(-1234.5, LAM, DX, n)
+    b = asm.transfer_exact(-777.25, LAM, DX, n)
+    c = asm.transfer_exact(-1234.5 - 777.25, LAM, DX, n)
+    np.testing.assert_allclose(a * b, c, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------------------
+# brute-force O(N^4) DFT propagation
+# ---------------------------------------------------------------------------------------
+def _dft_propagate(psi, z):
+    ny, nx = psi.shape
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    out = np.zeros_like(psi, dtype=np.complex128)
+    spec = np.zeros_like(out)
+    for ky in range(ny):
+        for kx in range(nx):
+            spec[ky, kx] = np.sum(psi * np.exp(-2j * np.pi * (kx * ii / nx + ky * jj / ny)))
+    for ky in range(ny):
+        for kx in range(nx):
+            fx = (kx if kx < nx - nx // 2 else kx - nx) / (nx * DX)
+            fy = (ky if ky < ny - ny // 2 else ky - ny) / (ny * DX)
+            d = 1.0 - (LAM * fx) ** 2 - (LAM * fy) ** 2
+            spec[ky, kx] *= np.exp(2j * np.pi * z / LAM * math.sqrt(d))
+    for j in range(ny):
+        for i in range(nx):
+            out[j, i] = np.sum(spec * np.exp(2j * np.pi * (ii * i / nx + jj * j / ny))) / (nx * ny)
+    return out
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_propagate_matches_bruteforce_dft(n):
+    rng = np.random.default_rng(n)
+    psi = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    z = -3000.0
+    want = _dft_propagate(psi, z)
+    got = asm.asm_prop(psi, z, LAM, DX)
+    np.testing.assert_allclose(got, want, atol=1e-10 * np.abs(want).max())
+
+
+# ---------------------------------------------------------------------------------------
+# closed forms of the propagation
+# ---------------------------------------------------------------------------------------
+def test_constant_field_is_invariant():
+    n = 32
+    psi = np.ones((n, n), complex)
+    out = asm.asm_prop(psi, -5e4, LAM, DX)
+    np.testing.assert_allclose(np.abs(out) ** 2, 1.0, atol=1e-12)
+    proj = asm.reconst_xyprojection(psi, 7, "exact", z_front=-1e4, z_dz=-1e3, wavlen=LAM, dx=DX)
+    np.testing.assert_allclose(proj, 1.0, atol=1e-6)
+
+
+def _grating_intensity(n, m, a, z):
+    f = m / (n * DX)
+    delta = 2 * np.pi * z / LAM * (math.sqrt(1 - (LAM * f) ** 2) - 1.0)
+    c = np.cos(2 * np.pi * m * np.arange(n) / n)
+    field = 1.0 + a * c * np.exp(1j * delta)
+    return np.broadcast_to(np.abs(field) ** 2, (n, n))
+
+
+def test_grating_closed_form_projection():
+    n, m, a = 64, 5, 0.3
+    psi = np.broadcast_to(1.0 + a * np.cos(2 * np.pi * m * np.arange(n) / n), (n, n)).astype(complex)
+    z0, dz, s = -1e4, -750.0, 12
+    want = np.min([_grating_intensity(n, m, a, z0 + k * dz) for k in range(s)], axis=0)
+    got = asm.reconst_xyprojection(psi, s, "exact", z_front=z0, z_dz=dz, wavlen=LAM, dx=DX)
+    np.testing.assert_allclose(got, want, atol=2e-7)
+
+
+def test_delta_constant_pair():
+    n = 16
+    delta = np.zeros((n, n), complex)
+    delta[0, 0] = 1.0
+    np.testing.assert_allclose(asm.fft2(delta), np.ones((n, n)), atol=0)
+    np.testing.assert_allclose(asm.ifft2(np.ones((n, n))), delta, atol=1e-15)
+
+
+def test_reversibility_and_parseval():
+    n = 64
+    rng = np.random.default_rng(3)
+    psi = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    fwd = asm.asm_prop(psi, 2500.0, LAM, DX)
+    back = asm.asm_prop(fwd, -2500.0, LAM, DX)
+    np.testing.assert_allclose(back, psi, atol=1e-11)
+    assert abs(np.sum(np.abs(fwd) ** 2) - np.sum(np.abs(psi) ** 2)) < 1e-9 * np.sum(np.abs(psi) ** 2)
+
+
+def test_reference_mode_running_product_equals_exact_at_small_depth():
+    # H0 .* Hdz^(k-1) == H(z0 + (k-1) dz): the reference's running product converges to the
+    # exact slice fields when the F32 phase defect is small (short distances, few slices).
+    n = 64
+    rng = np.random.default_rng(5)
+    holo = (1 + 0.2 * rng.standard_normal((n, n))).clip(0).astype(np.float32)
+    wf = asm.gabor_wavefront(holo)
+    d32 = asm.transfer_sqrt_arr(n, LAM, DX)
+    h0, hdz = asm.transfer_reference(-200.0, LAM, d32), asm.transfer_reference(-50.0, LAM, d32)
+    ref = asm.reconst_xyprojection(wf, 6, "reference", h_front=h0, h_dz=hdz)
+    ex = asm.reconst_xyprojection(wf, 6, "exact", z_front=-200.0, z_dz=-50.0, wavlen=LAM, dx=DX)
+    assert asm.rel_l2(ref, ex) < 1e-5
+
+
+def test_volume_slices_equal_single_propagations():
+    n = 32
+    rng = np.random.default_rng(9)
+    wf = asm.gabor_wavefront((1 + 0.3 * rng.random((n, n))).astype(np.float32))
+    cv = asm.reconst_complex_vol(wf, 3, "exact", z_front=-1e4, z_dz=-300.0, wavlen=LAM, dx=DX)
+    for k in range(3):
+        np.testing.assert_allclose(cv[k], asm.asm_prop(wf, -1e4 - 300.0 * k, LAM, DX), atol=1e-10)
+
+
+# ---------------------------------------------------------------------------------------
+# epilogues
+# ---------------------------------------------------------------------------------------
+def test_n0f8_rounding_ties_to_even():
+    # v*255 = k + 1/2 exactly: round(UInt8, .) rounds half to even
+    v = np.array([0.5 / 255, 1.5 / 255, 2.5 / 255, 254.5 / 255, -1.0, 2.0, 1.0, 0.0], np.float64)
+    got = asm.n0f8(v)
+    assert list(got) == [0, 2, 2, 254, 0, 255, 255, 0]
+
+
+def test_projection_from_volume_nan_rule():
+    vol = np.array([[[np.nan]], [[0.5]], [[0.25]]], np.float32)     # NaN first: sticks
+    assert np.isnan(asm.xy_projection_from_vol(vol)[0, 0])
+    vol = np.array([[[0.5]], [[np.nan]], [[0.25]]], np.float32)     # NaN later: ignored
+    assert asm.xy_projection_from_vol(vol)[0, 0] == np.float32(0.25)
+    u8 = np.array([[[200]], [[13]], [[90]]], np.uint8)
+    assert asm.xy_projection_from_vol(u8)[0, 0] == np.float32(13) / np.float32(255)
+
+
+# ---------------------------------------------------------------------------------------
+# f2: cu_2d_pad known-answer test (runtests.jl:362-368)
+# ---------------------------------------------------------------------------------------
+def test_pad2_known_answer():
+    arr = np.array([[1 + 1j, 2 + 1j], [3 + 1j, 4 + 1j]], np.complex64)
+    padded = asm.pad2(arr)
+    assert padded.shape == (4, 4)
+    np.testing.assert_array_equal(padded[1:3, 1:3], arr)
+    assert padded[0, 0] == np.complex64(2.5 + 1j)
+
+
+# ---------------------------------------------------------------------------------------
+# A10: Gerchberg-Saxton
+# ---------------------------------------------------------------------------------------
+def test_gs_plane_wave_fixed_point():
+    n = 16
+    i1 = np.full((n, n), 0.81, np.float32)
+    out = asm.phase_retrieval(i1, i1, 5, z=2000.0, wavlen=LAM, dx=DX)
+    np.testing.assert_allclose(out, np.sqrt(np.float32(0.81)) * np.exp(1j * np.angle(out[0, 0])), atol=1e-12)
+    # the phase of a plane wave after +z then -z is 0 again
+    np.testing.assert_allclose(np.angle(out), 0.0, atol=1e-12)
+
+
+def test_gs_amplitude_and_error_reduction():
+    n = 32
+    rng = np.random.default_rng(11)
+    psi_true = np.exp(1j * 0.5 * rng.standard_normal((n, n))) * (1 + 0.2 * rng.random((n, n)))
+    z = 3000.0
+    i1 = (np.abs(psi_true) ** 2).astype(np.float32)
+    i2 = (np.abs(asm.asm_prop(psi_true, z, LAM, DX)) ** 2).astype(np.float32)
+    hf = asm.natural(asm.transfer_exact(z, LAM, DX, n))
+    errs = []
+    for it in range(1, 8):
+        out = asm.phase_retrieval(i1, i2, it, z=z, wavlen=LAM, dx=DX)
+        np.testing.assert_allclose(np.abs(out), np.sqrt(i1.astype(np.float64)), rtol=1e-6)
+        errs.append(asm.gs_error(out, i2, hf))
+    assert all(b <= a * (1 + 1e-9) for a, b in zip(errs, errs[1:])), errs
+
+
+def test_gs_one_iteration_bruteforce():
+    n = 8
+    rng = np.random.default_rng(13)
+    i1 = (1 + 0.5 * rng.random((n, n))).astype(np.float32)
+    i2 = (1 + 0.5 * rng.random((n, n))).astype(np.float32)
+    z = 1500.0
+    a1 = np.sqrt(i1.astype(np.float64))
+    a2 = np.sqrt(i2.astype(np.float64))
+    p2 = _dft_propagate(a1.astype(complex), z)
+    p2 = a2 * p2 / np.abs(p2)
+    p1 = _dft_propagate(p2, -z)
+    p1 = a1 * p1 / np.abs(p1)
+    got = asm.phase_retrieval(i1, i2, 1, z=z, wavlen=LAM, dx=DX)
+    np.testing.assert_allclose(got, p1, atol=1e-10)
+
+
+def test_weighted_phase_rms_metric():
+    psi = np.exp(1j * np.array([[0.1, -0.2]]))
+    ref = np.ones((1, 2), complex)
+    assert abs(asm.weighted_phase_rms(psi, ref) - math.sqrt((0.01 + 0.04) / 2)) < 1e-12
