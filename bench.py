@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: z-slices/s of the fused angular-spectrum reconstruction + xy min-projection.
+
+Workload (the 1024^2 GPU configuration): one 1024 x 1024 Float32 inline hologram of a seeded
+1000-disk scene (depths 1e4-2e5 um, lambda = 0.6328 um, pitch 10 um), reconstructed at 1000
+slices from z0 = 1e4 um in steps of 200 um; complex-FP32 FFTs, FP64-wrapped transfer phase,
+fused multiply / inverse FFT / |.|^2 / running min. One step = one full reconstruction
+(sqrt + forward spectrum + 1000 slices) of the device-resident hologram.
+
+Prints one JSON line. --gpus N > 1 is refused: this build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "particleholography.jl_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "z-slices/s at 1024² CF32, % of HBM roofline; p50/p99 per-hologram latency"
+HBM_PEAK_GBS = 8000.0
+LAM, DX = 0.6328, 10.0
+N, SLICES, Z0, DZ, NDISKS, SEED = 1024, 1000, 1e4, 200.0, 1000, 102
+PASS_BYTES_PER_PX = 16.0  # per slice: pass A reads the spectrum (8) and writes the column
+                          # transform (8); pass B reads it (8) and read-modify-writes the
+                          # projection (4 + 4). 32 B/px per slice for the pair.
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_steps(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    for _ in range(steps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, e0.elapsed_time(e1) / 1e3
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the plain-path and GS figures")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the z-range "
+              "sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import particleholography as ph
+    from particleholography import _lib, synth
+
+    dev = torch.device("cuda:0")
+    log(f"bench: generating the {N}^2 hologram (seed {SEED}, {NDISKS} disks)")
+    holo_np = synth.hologram(SEED, NDISKS, N, zmin=1e4, zmax=2e5, wavlen=LAM, dx=DX)
+    holo = torch.from_numpy(holo_np).to(dev)
+    d_sqr = ph.cu_transfer_sqrt_arr(N, LAM, DX)
+    h0 = ph.cu_transfer(-Z0, N, LAM, d_sqr)
+    hdz = ph.cu_transfer(-DZ, N, LAM, d_sqr)
+    proj = torch.empty((N, N), dtype=torch.float32, device=dev)
+    hd = _lib.handle(0)
+
+    def step():
+        ph.reconst_xyprojection_holo(holo, h0, hdz, SLICES, out=proj)
+
+    log(f"bench: {args.warmup} warm-up + {args.steps} timed steps")
+    wall, gpu_s = timed_steps(step, args.steps, args.warmup, torch)
+    slices_per_s = args.steps * SLICES / wall
+    ms_per_step = wall / args.steps * 1e3
+    log(f"bench: {slices_per_s:.0f} slices/s ({ms_per_step:.3f} ms/step, events {gpu_s / args.steps * 1e3:.3f} ms)")
+
+    # per-pass HIP-event timing over one more step (same kernels, same stream)
+    hd.set_profiling(True)
+    for p in range(5):
+        hd.profile_read(p, reset=True)
+    step()
+    torch.cuda.synchronize()
+    passes = {}
+    for pid, name in ((_lib.HOLO_PASS_RECON_COLS, "recon_cols"), (_lib.HOLO_PASS_RECON_ROWS, "recon_rows"),
+                      (_lib.HOLO_PASS_FWD_FFT, "forward_fft")):
+        ms, n, units = hd.profile_read(pid)
+        passes[name] = {"ms_total": ms, "launches": n, "slices": units}
+    hd.set_profiling(False)
+    dom = max(("recon_cols", "recon_rows"), key=lambda k: passes[k]["ms_total"])
+    d = passes[dom]
+    alg_bytes = PASS_BYTES_PER_PX * N * N * d["slices"]
+    achieved = alg_bytes / (d["ms_total"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_recon_cols (pass A)" if dom == "recon_cols" else "k_recon_rows_proj (pass B)",
+                "avg_launch_ms": round(d["ms_total"] / max(d["launches"], 1), 4),
+                "bytes_per_launch": alg_bytes / max(d["launches"], 1)}
+    extras = {"passes": passes, "step_algorithmic_GBs": round(32.0 * N * N * slices_per_s / 1e9, 1)}
+
+    if not args.no_extras:
+        # the plain one-slice-at-a-time GPU path: one pass-A + pass-B launch pair per slice
+        hd.set_tuning(1, 1, 1, 0, 0)
+        try:
+            pw, _ = timed_steps(step, 2, 1, torch)
+        finally:
+            hd.set_tuning()
+        plain = 2 * SLICES / pw
+        extras["plain_path_slices_per_s"] = round(plain, 1)
+        extras["speedup_vs_plain"] = round(slices_per_s / plain, 2)
+        # Gerchberg-Saxton at 1024^2: 100 iterations, hipGraph-captured iteration
+        i1 = holo
+        i2 = torch.from_numpy(synth.gs_pair(103, 500, N, 2000.0)[1]).to(dev)
+        hf, hb = ph.cu_transfer(2000.0, N, LAM, d_sqr), ph.cu_transfer(-2000.0, N, LAM, d_sqr)
+        gs = lambda: ph.cu_phase_retrieval_holo(i1, i2, hf, hb, 100, N)  # noqa: E731
+        gw, _ = timed_steps(gs, 3, 1, torch)
+        extras["gs_iterations_per_s"] = round(300 / gw, 1)
+        extras["gs_algorithmic_GBs"] = round(72.0 * N * N * 300 / gw / 1e9, 1)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import cpu_baseline
+
+        log("bench: CPU baseline (reference op sequence, torch CPU complex64)")
+        cpu = cpu_baseline.measure(holo_np, -Z0, -DZ, LAM, DX, SLICES, budget_s=15.0)
+        cpu["value"] = round(cpu["value"], 2)
+
+    out = {
+        "metric": METRIC, "value": round(slices_per_s, 1), "unit": "z-slices/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "cf32",
+        "data": "synthetic",
+        "config": {"workload": "1024x1024 Float32 hologram, 1000-disk seeded scene, 1000 slices "
+                               "at dz=200 um from z0=1e4 um, xy min-projection", "size": [N, N],
+                   "slices": SLICES, "z0_um": Z0, "dz_um": DZ, "wavelength_um": LAM, "pitch_um": DX,
+                   "transfer": "generated FP64-wrapped phase", "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "extras": extras,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

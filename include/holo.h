@@ -147,6 +147,24 @@ int holo_asm_prop(holo_handle h, void* stream, int nx, int ny, void* out, const 
 int holo_fft2(holo_handle h, void* stream, int nx, int ny, const void* in, void* out,
               int inverse);
 
+/* ---- measurement (no counterpart in the reference, which has no timers) ---------------- */
+/* Passes timed with HIP events when profiling is on: the fused reconstruction's column pass
+ * (pass A) and row pass (pass B), the forward spectrum, the GS propagation column pass and
+ * the GS row pass. Units: slices for the reconstruction passes, launches otherwise. Events
+ * are recorded on the caller's stream around each launch (not inside graph captures). */
+enum holo_pass {
+  HOLO_PASS_RECON_COLS = 0,
+  HOLO_PASS_RECON_ROWS = 1,
+  HOLO_PASS_FWD_FFT = 2,
+  HOLO_PASS_PROP_COLS = 3,
+  HOLO_PASS_GS_ROWS = 4
+};
+int holo_set_profiling(holo_handle h, int enable);
+/* Waits for the recorded events; returns the accumulated time (ms), launches and units of a
+ * pass since the last reset. */
+int holo_profile_read(holo_handle h, int pass, double* total_ms, long long* launches,
+                      long long* units, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,69 @@
+"""CPU baseline: the reference's own op sequence for cu_get_reconst_xyprojection, on host cores.
+
+Test / benchmark infrastructure only (the CPU-baseline leg of bench.py). The reference has no
+CPU path (every function is typed on CuArray), so this restates holofunc.jl:254-263 on
+Array inputs with the same arithmetic: complex64 FFTs (torch.fft on CPU, multithreaded),
+fftshift / ifftshift, the ComplexF32 running product ``F .*= Hdz`` and
+``min(proj, abs2(ifft(ifftshift(F))))``; transfer functions in the reference's arithmetic
+(Float32 d and sqrt, FP64 exp -> ComplexF32, holofunc.jl:8-15, 38-45).
+"""
+from __future__ import annotations
+
+import os
+import time
+
+import numpy as np
+
+from . import asm
+
+
+def cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def reconst_xyprojection_cpu(holo: np.ndarray, h0: np.ndarray, hdz: np.ndarray, slices: int):
+    """holofunc.jl:80-82 + 254-263 with torch CPU complex64."""
+    import torch
+
+    psi = torch.from_numpy(np.sqrt(holo.astype(np.float32))).to(torch.complex64)
+    H0 = torch.from_numpy(h0.astype(np.complex64))
+    Hdz = torch.from_numpy(hdz.astype(np.complex64))
+    F = torch.fft.fftshift(torch.fft.fft2(psi)) * H0
+    x = torch.fft.ifft2(torch.fft.ifftshift(F))
+    proj = (x * x.conj()).abs().to(torch.float32)
+    for _ in range(1, slices):
+        F *= Hdz
+        x = torch.fft.ifft2(torch.fft.ifftshift(F))
+        proj = torch.fmin(proj, (x * x.conj()).abs().to(torch.float32))
+    return proj.numpy()
+
+
+def measure(holo: np.ndarray, z0: float, dz: float, wavlen: float, dx: float, slices: int,
+            budget_s: float = 15.0, threads: int | None = None) -> dict:
+    """Times the CPU restatement on ``threads`` host threads (default: every core in the
+    affinity mask). The sample is bounded: the slice count is cut so one run takes about
+    ``budget_s / 3`` seconds; returns slices/s over the median of 3 runs."""
+    import torch
+
+    n_threads = threads or cores()
+    torch.set_num_threads(n_threads)
+    ny, nx = holo.shape
+    d32 = asm.transfer_sqrt_arr(nx, wavlen, dx, ny)
+    h0 = asm.transfer_reference(z0, wavlen, d32)
+    hdz = asm.transfer_reference(dz, wavlen, d32)
+    t = time.perf_counter()
+    reconst_xyprojection_cpu(holo, h0, hdz, 2)  # warm-up, also sizes the sample
+    per_slice = max((time.perf_counter() - t) / 2.0, 1e-4)
+    sample = int(max(2, min(slices, (budget_s / 3.0) / per_slice)))
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        reconst_xyprojection_cpu(holo, h0, hdz, sample)
+        times.append(time.perf_counter() - t)
+    med = float(np.median(times))
+    return {"value": sample / med, "unit": "z-slices/s", "cores": n_threads, "kind": "port",
+            "sample": f"{nx}x{ny} hologram, {sample} of {slices} slices, median of 3 runs, "
+                      f"torch {torch.__version__} CPU complex64 FFT on {n_threads} threads"}

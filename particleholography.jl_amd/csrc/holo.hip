@@ -105,9 +105,60 @@ struct holo_context {
     }
   };
   std::map<GsKey, hipGraphExec_t> gs_graphs;
+  // per-pass HIP-event timing (holo_set_profiling)
+  struct ProfRec {
+    int pass;
+    hipEvent_t a, b;
+    long long units;
+  };
+  bool prof = false;
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> ev_pool;
+  double prof_ms[8] = {};
+  long long prof_launch[8] = {}, prof_units[8] = {};
 };
 
 namespace {
+
+hipEvent_t ev_get(holo_context* h) {
+  if (!h->ev_pool.empty()) {
+    hipEvent_t e = h->ev_pool.back();
+    h->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Opens a timed region on `st` when profiling is on (never inside a graph capture).
+hipEvent_t prof_begin(holo_context* h, hipStream_t st) {
+  if (!h->prof || st == h->cap) return nullptr;
+  hipEvent_t a = ev_get(h);
+  if (a && hipEventRecord(a, st) != hipSuccess) return nullptr;
+  return a;
+}
+
+void prof_end(holo_context* h, hipStream_t st, hipEvent_t a, int pass, long long units) {
+  if (!a) return;
+  hipEvent_t b = ev_get(h);
+  if (!b || hipEventRecord(b, st) != hipSuccess) return;
+  h->prof_pending.push_back({pass, a, b, units});
+}
+
+void prof_collect(holo_context* h) {
+  for (auto& r : h->prof_pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      h->prof_ms[r.pass] += ms;
+      h->prof_launch[r.pass] += 1;
+      h->prof_units[r.pass] += r.units;
+    }
+    h->ev_pool.push_back(r.a);
+    h->ev_pool.push_back(r.b);
+  }
+  h->prof_pending.clear();
+}
 
 int get_plan(holo_context* h, int L, const Plan1D** out) {
   auto it = h->plans.find(L);
@@ -352,7 +403,11 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
   float2* run = G + plane * rp.m;
   const size_t oplane = static_cast<size_t>(onx) * ony;
   if (proj) HOLO_TRY(launch_fill(proj, oplane, 0x7fffffffu, st));
-  HOLO_TRY(forward_spectrum(g, Ex, Ey, src, src_real, F, st));
+  {
+    hipEvent_t pe = prof_begin(h, st);
+    HOLO_TRY(forward_spectrum(g, Ex, Ey, src, src_real, F, st));
+    prof_end(h, st, pe, HOLO_PASS_FWD_FFT, 1);
+  }
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
   const size_t clds = col_lds(g), rlds = row_lds(g);
   for (int b0 = 0; b0 < slices; b0 += rp.m) {
@@ -374,6 +429,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
       ra.run = run;
       ra.first = b0 == 0;
     }
+    hipEvent_t pa = prof_begin(h, st);
     bool ok = dispatch_E(Ey, [&](auto ec) {
       constexpr int E = decltype(ec)::value;
       if (!arrays) {
@@ -386,7 +442,9 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     });
     if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
     HOLO_TRY(check_launch());
+    prof_end(h, st, pa, HOLO_PASS_RECON_COLS, nb);
     const int ry = (nb + rp.spg_r - 1) / rp.spg_r;
+    hipEvent_t pb = prof_begin(h, st);
     ok = dispatch_E(Ex, [&](auto ec) {
       constexpr int E = decltype(ec)::value;
       if (kind == 0) {
@@ -422,6 +480,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     });
     if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
     HOLO_TRY(check_launch());
+    prof_end(h, st, pb, HOLO_PASS_RECON_ROWS, nb);
   }
   return HOLO_OK;
 }
@@ -471,13 +530,19 @@ int launch_gs_rows(const Geom& g, int Ex, int mode, const float2* U, const float
   return check_launch();
 }
 
-int gs_iteration(const Geom& g, int Ex, int Ey, float2* B, const float* I1, const float* I2,
-                 const holo_tf* fwd, const holo_tf* bwd, float sc, bool last,
+int gs_iteration(holo_context* h, const Geom& g, int Ex, int Ey, float2* B, const float* I1,
+                 const float* I2, const holo_tf* fwd, const holo_tf* bwd, float sc, bool last,
                  float2* out, hipStream_t st) {
-  HOLO_TRY(launch_prop_cols(g, Ey, fwd, B, B, st));
-  HOLO_TRY(launch_gs_rows(g, Ex, 0, B, I2, B, sc, st));
-  HOLO_TRY(launch_prop_cols(g, Ey, bwd, B, B, st));
-  return launch_gs_rows(g, Ex, last ? 1 : 0, B, I1, last ? out : B, sc, st);
+  for (int half = 0; half < 2; ++half) {
+    hipEvent_t pc = prof_begin(h, st);
+    HOLO_TRY(launch_prop_cols(g, Ey, half == 0 ? fwd : bwd, B, B, st));
+    prof_end(h, st, pc, HOLO_PASS_PROP_COLS, 1);
+    hipEvent_t pr = prof_begin(h, st);
+    if (half == 0) HOLO_TRY(launch_gs_rows(g, Ex, 0, B, I2, B, sc, st));
+    else HOLO_TRY(launch_gs_rows(g, Ex, last ? 1 : 0, B, I1, last ? out : B, sc, st));
+    prof_end(h, st, pr, HOLO_PASS_GS_ROWS, 1);
+  }
+  return HOLO_OK;
 }
 
 }  // namespace
@@ -525,6 +590,8 @@ int holo_destroy(holo_handle h) {
   (void)hipSetDevice(h->device);
   (void)hipDeviceSynchronize();
   for (auto& kv : h->gs_graphs) (void)hipGraphExecDestroy(kv.second);
+  prof_collect(h);
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   for (auto& kv : h->plans) (void)hipFree(kv.second.dtw);
   if (h->ws) (void)hipFree(h->ws);
   if (h->cap) (void)hipStreamDestroy(h->cap);
@@ -711,7 +778,7 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
     } else {
       hipGraph_t graph = nullptr;
       HOLO_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
-      const int r = gs_iteration(g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, false, o, h->cap);
+      const int r = gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, false, o, h->cap);
       const hipError_t e = hipStreamEndCapture(h->cap, &graph);
       if (r != HOLO_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -726,9 +793,30 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
     for (int it2 = 0; it2 < body; ++it2) HOLO_HIP(hipGraphLaunch(exec, st));
   } else {
     for (int it2 = 0; it2 < body; ++it2)
-      HOLO_TRY(gs_iteration(g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, false, o, st));
+      HOLO_TRY(gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, false, o, st));
   }
-  return gs_iteration(g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, true, o, st);
+  return gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, true, o, st);
+}
+
+int holo_set_profiling(holo_handle h, int enable) {
+  if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
+  h->prof = enable != 0;
+  return HOLO_OK;
+}
+
+int holo_profile_read(holo_handle h, int pass, double* total_ms, long long* launches,
+                      long long* units, int reset) {
+  if (!h || pass < 0 || pass >= 8) return fail(HOLO_ERR_ARG, "bad handle or pass");
+  prof_collect(h);
+  if (total_ms) *total_ms = h->prof_ms[pass];
+  if (launches) *launches = h->prof_launch[pass];
+  if (units) *units = h->prof_units[pass];
+  if (reset) {
+    h->prof_ms[pass] = 0.0;
+    h->prof_launch[pass] = 0;
+    h->prof_units[pass] = 0;
+  }
+  return HOLO_OK;
 }
 
 }  // extern "C"

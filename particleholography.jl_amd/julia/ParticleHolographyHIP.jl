@@ -1,0 +1,205 @@
+# ParticleHolographyHIP.jl -- the reference-side binding: ParticleHolography.jl's holofunc.jl API
+# (src/holofunc.jl:5-6 exports, src/types.jl:3) on AMDGPU.jl ROCArrays, over libholo's C ABI
+# (include/holo.h) with `ccall`. Drop-in: same names, argument order and return types; the
+# structs keep their `.data` field and 1-argument constructors (runtests.jl:10-60) and gain
+# optional provenance (z, wavelength, pitch) that lets libholo generate H in FP64.
+module ParticleHolographyHIP
+
+using AMDGPU
+using FixedPointNumbers
+
+export CuTransferSqrtPart, CuTransfer, CuWavefront
+export cu_transfer_sqrt_arr, cu_transfer, cu_gabor_wavefront, cu_phase_retrieval_holo,
+       cu_get_reconst_vol, cu_get_reconst_xyprojection, cu_get_reconst_vol_and_xyprojection,
+       cu_get_reconst_complex_vol, cu_asm_prop!, cu_2d_pad, cu_get_reconst_vol_and_xyprojection_padded
+
+const libholo = get(ENV, "HOLO_LIB", joinpath(@__DIR__, "..", "lib", "libholo.so"))
+
+# ---- types.jl:13-61 ------------------------------------------------------------------------
+struct CuTransferSqrtPart{T<:AbstractFloat}
+    data::ROCArray{T,2}
+    wavlen::Union{Nothing,Float64}
+    dx::Union{Nothing,Float64}
+end
+CuTransferSqrtPart(data::ROCArray{T,2}) where {T} = CuTransferSqrtPart{T}(data, nothing, nothing)
+
+struct CuTransfer{T<:Complex}
+    data::ROCArray{T,2}
+    z::Union{Nothing,Float64}
+    wavlen::Union{Nothing,Float64}
+    dx::Union{Nothing,Float64}
+end
+CuTransfer(data::ROCArray{T,2}) where {T} = CuTransfer{T}(data, nothing, nothing, nothing)
+
+struct CuWavefront{T<:Complex}
+    data::ROCArray{T,2}
+end
+
+for T in (CuTransferSqrtPart, CuTransfer, CuWavefront)
+    @eval begin
+        Base.size(x::$T) = size(x.data)
+        Base.axes(x::$T) = axes(x.data)
+        Base.eltype(x::$T) = eltype(x.data)
+        Base.ndims(x::$T) = ndims(x.data)
+    end
+end
+
+# ---- C ABI plumbing ----------------------------------------------------------------------------
+struct HoloTF            # holo_tf
+    data::Ptr{Cvoid}
+    z::Float64
+    wavlen::Float64
+    dx::Float64
+    generated::Cint
+end
+
+const HOLO_N0F8, HOLO_F16, HOLO_F32, HOLO_F64, HOLO_C64 = 0, 1, 2, 3, 4
+const _handles = Dict{Int,Ptr{Cvoid}}()
+
+function _handle()
+    dev = AMDGPU.device_id(AMDGPU.device()) - 1
+    get!(_handles, dev) do
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        _check(ccall((:holo_create, libholo), Cint, (Ptr{Ptr{Cvoid}}, Cint), r, dev))
+        r[]
+    end
+end
+
+_stream() = AMDGPU.stream().stream   # hipStream_t of the task-local stream
+
+function _check(code::Cint)
+    code == 0 && return nothing
+    msg = unsafe_string(ccall((:holo_last_error, libholo), Cstring, ()))
+    code == 1 && throw(AssertionError(msg))
+    code == 3 && throw(ArgumentError(msg))
+    error("libholo error $code: $msg")
+end
+
+_generated(t::CuTransfer) = t.z !== nothing && t.wavlen !== nothing && t.dx !== nothing
+_tf(t::CuTransfer, gen::Bool) = gen ? HoloTF(C_NULL, t.z, t.wavlen, t.dx, 1) :
+                                      HoloTF(Ptr{Cvoid}(pointer(t.data)), 0.0, 0.0, 0.0, 0)
+_dtype(::Type{N0f8}) = HOLO_N0F8
+_dtype(::Type{Float16}) = HOLO_F16
+_dtype(::Type{Float32}) = HOLO_F32
+_dtype(::Type{Float64}) = HOLO_F64
+_dtype(T) = throw(ArgumentError("return_type must be Subtype of AbstractFloat or N0f8. Got $T"))
+
+# ---- holofunc.jl -------------------------------------------------------------------------------
+function cu_transfer_sqrt_arr(datlen::Int, wavlen::AbstractFloat, dx::AbstractFloat)
+    plane = ROCArray{Float32}(undef, datlen, datlen)
+    _check(ccall((:holo_transfer_sqrt, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cdouble, Cdouble, Cint, Ptr{Cvoid}),
+                 _handle(), _stream(), datlen, datlen, wavlen, dx, HOLO_F32, pointer(plane)))
+    return CuTransferSqrtPart{Float32}(plane, Float64(wavlen), Float64(dx))
+end
+
+function cu_transfer(z0::AbstractFloat, datLen::Int, wavLen::AbstractFloat, d_sqr::CuTransferSqrtPart)
+    plane = ROCArray{ComplexF32}(undef, datLen, datLen)
+    prov = d_sqr.dx !== nothing && d_sqr.wavlen == Float64(wavLen)
+    mode = prov ? 2 : 1                       # HOLO_TF_FP64 / HOLO_TF_FP64_D
+    dtype = eltype(d_sqr.data) == Float32 ? HOLO_F32 : HOLO_F64
+    _check(ccall((:holo_transfer, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cdouble, Cdouble, Ptr{Cvoid}, Cint, Cint, Cdouble, Ptr{Cvoid}),
+                 _handle(), _stream(), datLen, datLen, z0, wavLen, pointer(d_sqr.data), dtype, mode,
+                 prov ? d_sqr.dx : 0.0, pointer(plane)))
+    return prov ? CuTransfer{ComplexF32}(plane, Float64(z0), Float64(wavLen), d_sqr.dx) : CuTransfer(plane)
+end
+
+function cu_gabor_wavefront(holo::ROCArray{Float32,2})
+    out = ROCArray{ComplexF32}(undef, size(holo)...)
+    _check(ccall((:holo_gabor_wavefront, libholo), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+                 _handle(), _stream(), size(holo, 1), size(holo, 2), pointer(holo), pointer(out)))
+    return CuWavefront(out)
+end
+
+function cu_phase_retrieval_holo(holo1::ROCArray{Float32,2}, holo2::ROCArray{Float32,2}, transfer::CuTransfer{ComplexF32},
+                                 invtransfer::CuTransfer{ComplexF32}, priter::Int, datlen::Int)
+    @assert size(holo1) == size(holo2) == size(transfer.data) == size(invtransfer.data) == (datlen, datlen) "All arrays must have the same size as ($datlen, $datlen). Got $(size(holo1)), $(size(holo2)), $(size(transfer.data)), $(size(invtransfer.data))."
+    gen = _generated(transfer) && _generated(invtransfer)
+    out = ROCArray{ComplexF32}(undef, datlen, datlen)
+    _check(ccall((:holo_phase_retrieval, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ref{HoloTF}, Ref{HoloTF}, Cint, Ptr{Cvoid}, Cint),
+                 _handle(), _stream(), datlen, datlen, pointer(holo1), pointer(holo2),
+                 _tf(transfer, gen), _tf(invtransfer, gen), priter, pointer(out), 1))
+    return CuWavefront(out)
+end
+
+function cu_2d_pad(inarr)
+    nx, ny = size(inarr)
+    out = ROCArray{ComplexF32}(undef, 2nx, 2ny)
+    src = ROCArray{ComplexF32}(inarr)
+    _check(ccall((:holo_2d_pad, libholo), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+                 _handle(), _stream(), nx, ny, pointer(src), pointer(out)))
+    return out
+end
+
+function _sizes_ok(wavefront, tf_front, tf_dz)
+    @assert size(wavefront.data) == size(tf_front.data) == size(tf_dz.data) "All arrays must have the same size. Got $(size(wavefront.data)), $(size(tf_front.data)), $(size(tf_dz.data))."
+end
+
+function cu_get_reconst_xyprojection(wavefront::CuWavefront{ComplexF32}, transfer_front::CuTransfer{ComplexF32},
+                                     transfer_dz::CuTransfer{ComplexF32}, slices::Int)
+    _sizes_ok(wavefront, transfer_front, transfer_dz)
+    gen = _generated(transfer_front) && _generated(transfer_dz)
+    nx, ny = size(wavefront.data)
+    proj = ROCArray{Float32}(undef, nx, ny)
+    _check(ccall((:holo_reconst_xyprojection, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ref{HoloTF}, Ref{HoloTF}, Cint, Ptr{Cvoid}),
+                 _handle(), _stream(), nx, ny, pointer(wavefront.data), _tf(transfer_front, gen),
+                 _tf(transfer_dz, gen), slices, pointer(proj)))
+    return proj
+end
+
+function _reconst_vol(wavefront, transfer_front, transfer_dz, slices, T, code, with_proj)
+    _sizes_ok(wavefront, transfer_front, transfer_dz)
+    gen = _generated(transfer_front) && _generated(transfer_dz)
+    nx, ny = size(wavefront.data)
+    vol = ROCArray{T}(undef, nx, ny, slices)
+    proj = with_proj ? ROCArray{Float32}(undef, nx, ny) : nothing
+    _check(ccall((:holo_reconst_vol, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ref{HoloTF}, Ref{HoloTF}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+                 _handle(), _stream(), nx, ny, pointer(wavefront.data), _tf(transfer_front, gen), _tf(transfer_dz, gen),
+                 slices, code, pointer(vol), with_proj ? pointer(proj) : C_NULL))
+    return vol, proj
+end
+
+cu_get_reconst_vol(wavefront::CuWavefront{ComplexF32}, transfer_front::CuTransfer{ComplexF32},
+                   transfer_dz::CuTransfer{ComplexF32}, slices::Int, return_type::Type=N0f8) =
+    _reconst_vol(wavefront, transfer_front, transfer_dz, slices, return_type, _dtype(return_type), false)[1]
+
+cu_get_reconst_complex_vol(wavefront::CuWavefront{ComplexF32}, transfer_front::CuTransfer{ComplexF32},
+                           transfer_dz::CuTransfer{ComplexF32}, slices::Int) =
+    _reconst_vol(wavefront, transfer_front, transfer_dz, slices, ComplexF32, HOLO_C64, false)[1]
+
+cu_get_reconst_vol_and_xyprojection(wavefront::CuWavefront{ComplexF32}, transfer_front::CuTransfer{ComplexF32},
+                                    transfer_dz::CuTransfer{ComplexF32}, slices::Int, return_type::Type=N0f8) =
+    _reconst_vol(wavefront, transfer_front, transfer_dz, slices, return_type, _dtype(return_type), true)
+
+function cu_get_reconst_vol_and_xyprojection_padded(wavefront::CuWavefront{ComplexF32}, transfer_front::CuTransfer{ComplexF32},
+                                                    transfer_dz::CuTransfer{ComplexF32}, slices::Int, return_type::Type=N0f8)
+    expected_size = map(x -> 2 * x, size(wavefront.data))
+    @assert expected_size == size(transfer_front.data) == size(transfer_dz.data) "size(transfer_front.data) and size(transfer_dz.data) must be equal to 2*size(wavefront.data). Got $(size(wavefront.data)), $(size(transfer_front.data)), $(size(transfer_dz.data))."
+    _dtype(return_type)
+    gen = _generated(transfer_front) && _generated(transfer_dz)
+    nx, ny = size(wavefront.data)
+    vol = ROCArray{N0f8}(undef, nx, ny, slices)
+    proj = ROCArray{Float32}(undef, nx, ny)
+    _check(ccall((:holo_reconst_vol_and_xyprojection_padded, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ref{HoloTF}, Ref{HoloTF}, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+                 _handle(), _stream(), nx, ny, pointer(wavefront.data), _tf(transfer_front, gen), _tf(transfer_dz, gen),
+                 slices, pointer(vol), pointer(proj)))
+    return vol, proj
+end
+
+function cu_asm_prop!(outholo::CuWavefront, inholo::CuWavefront, d_sqr::CuTransferSqrtPart,
+                      zprop::Float64, datlen::Int, λ::Float64)
+    tf = d_sqr.dx !== nothing && d_sqr.wavlen == λ ?
+         HoloTF(C_NULL, zprop, λ, d_sqr.dx, 1) :
+         _tf(cu_transfer(zprop, datlen, λ, d_sqr), false)
+    _check(ccall((:holo_asm_prop, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ref{HoloTF}),
+                 _handle(), _stream(), datlen, datlen, pointer(outholo.data), pointer(inholo.data), tf))
+    return nothing
+end
+
+end # module

@@ -52,7 +52,11 @@ SIGNATURES = {
     "holo_phase_retrieval": [_vp, _vp, _i, _i, _vp, _vp, _tfp, _tfp, _i, _vp, _i],
     "holo_asm_prop": [_vp, _vp, _i, _i, _vp, _vp, _tfp],
     "holo_fft2": [_vp, _vp, _i, _i, _vp, _vp, _i],
+    "holo_set_profiling": [_vp, _i],
+    "holo_profile_read": [_vp, _i, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong),
+                          ctypes.POINTER(ctypes.c_longlong), _i],
 }
+HOLO_PASS_RECON_COLS, HOLO_PASS_RECON_ROWS, HOLO_PASS_FWD_FFT, HOLO_PASS_PROP_COLS, HOLO_PASS_GS_ROWS = range(5)
 
 _lib = None
 _lock = threading.Lock()
@@ -115,6 +119,16 @@ class Handle:
 
     def set_tuning(self, batch=0, spg_cols=0, spg_rows=0, cols_per_wg=0, rows_per_wg=0):
         check(self.lib.holo_set_tuning(self.ptr, batch, spg_cols, spg_rows, cols_per_wg, rows_per_wg))
+
+    def set_profiling(self, enable: bool):
+        check(self.lib.holo_set_profiling(self.ptr, 1 if enable else 0))
+
+    def profile_read(self, pass_id: int, reset: bool = True):
+        """(total_ms, launches, units) of one pass since the last reset."""
+        ms, n, u = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong()
+        check(self.lib.holo_profile_read(self.ptr, pass_id, ctypes.byref(ms), ctypes.byref(n),
+                                         ctypes.byref(u), 1 if reset else 0))
+        return ms.value, n.value, u.value
 
 
 _handles: dict = {}
