@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the plain-path and GS figures")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="for rocprofv3: one warm-up + --steps steps, then a 256 MiB device copy "
+                         "(a kernel of known traffic to scale FETCH_SIZE / WRITE_SIZE); no extras")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the z-range "
@@ -83,6 +86,17 @@ def main():
     def step():
         ph.reconst_xyprojection_holo(holo, h0, hdz, SLICES, out=proj)
 
+    if args.profile_run:
+        timed_steps(step, args.steps, 1, torch)
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        log("bench: profile run done (copy kernel: 268435456 B read, 268435456 B written)")
+        print(json.dumps({"profile_run": True, "steps": args.steps, "copy_bytes": 268435456}), flush=True)
+        return
+
     log(f"bench: {args.warmup} warm-up + {args.steps} timed steps")
     wall, gpu_s = timed_steps(step, args.steps, args.warmup, torch)
     slices_per_s = args.steps * SLICES / wall
@@ -105,8 +119,16 @@ def main():
     d = passes[dom]
     alg_bytes = PASS_BYTES_PER_PX * N * N * d["slices"]
     achieved = alg_bytes / (d["ms_total"] * 1e-3) / 1e9
+    kname = "k_recon_cols<" if dom == "recon_cols" else "k_recon_rows_proj<"
+    traffic, traffic_src = None, None
+    pmc_file = ROOT / "profiles" / "pmc_traffic_kernels.json"
+    if pmc_file.exists():  # HBM bytes per full launch from the committed rocprofv3 PMC runs
+        pmc = json.loads(pmc_file.read_text())
+        for kn, v in pmc["kernels"].items():
+            if kname in kn and f", {N}" in kn:
+                traffic, traffic_src = int(v["fetch_bytes_mean"] + v["write_bytes_mean"]), pmc_file.name
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "k_recon_cols (pass A)" if dom == "recon_cols" else "k_recon_rows_proj (pass B)",
                 "avg_launch_ms": round(d["ms_total"] / max(d["launches"], 1), 4),
                 "bytes_per_launch": alg_bytes / max(d["launches"], 1)}

@@ -31,11 +31,19 @@ struct StagePlan {
 };
 
 struct LinePlan {
-  int L;   // line length
-  int T;   // threads per line (L / E)
-  int ns;  // number of stages
+  int L;    // line length
+  int T;    // threads per line (L / E)
+  int ns;   // number of stages
+  int ntw;  // twiddle-table entries: sum over stages s >= 1 of (R_s - 1) * p_s = L - R_0
   StagePlan st[kMaxStages];
 };
+
+// Copies a plan's twiddle table into LDS (every thread of the workgroup participates). The
+// first stage needs no twiddles and ends with a barrier before its LDS writes, so the copy
+// is complete and visible before any stage reads it.
+__device__ __forceinline__ void stage_twiddles(float2* __restrict__ dst, const float2* __restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
 
 // ------------------------------------------------------------------------------------------
 // complex helpers
@@ -301,6 +309,118 @@ __device__ __forceinline__ void fft_line(float2 (&v)[E], float2* __restrict__ ld
       default: break;
     }
   }
+}
+
+// ------------------------------------------------------------------------------------------
+// Compile-time plans for the hot lengths: the same radix schedule and twiddle-table layout
+// as the host planner (largest radix dividing both E and the remaining length, first), with
+// every radix, span, stride and table offset a constant.
+// ------------------------------------------------------------------------------------------
+struct CPlanData {
+  int ns;
+  int R[kMaxStages];
+  int P[kMaxStages];
+  int off[kMaxStages];
+  int ntw;
+};
+
+constexpr CPlanData make_cplan(int L, int E) {
+  CPlanData d{};
+  const int radices[6] = {16, 8, 5, 4, 3, 2};
+  int rem = L, span = 1, off = 0;
+  while (rem > 1 && d.ns < kMaxStages) {
+    int R = 0;
+    for (int i = 0; i < 6 && R == 0; ++i)
+      if (E % radices[i] == 0 && rem % radices[i] == 0) R = radices[i];
+    if (R == 0) break;
+    d.R[d.ns] = R;
+    d.P[d.ns] = span;
+    d.off[d.ns] = off;
+    if (span > 1) off += (R - 1) * span;
+    ++d.ns;
+    span *= R;
+    rem /= R;
+  }
+  d.ntw = off;
+  return d;
+}
+
+template <int L, int E>
+struct CPlan {
+  static constexpr CPlanData d = make_cplan(L, E);
+  static constexpr int T = L / E;
+};
+
+template <int R, int E, int DIR, int T, int P, int OFF, bool LAST>
+__device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const float2* __restrict__ tw,
+                                             float2* __restrict__ lds, const int es) {
+  static_assert(E % R == 0, "radix must divide the per-thread element count");
+  constexpr int Q = E / R;
+  int base[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    float2 x[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) x[m] = v[q + Q * m];
+    const int b = t + T * q;
+    if constexpr (P > 1) {
+      const int j = b / P;
+      const int k = b - j * P;
+      const float2* __restrict__ w = tw + OFF + k;
+#pragma unroll
+      for (int m = 1; m < R; ++m) {
+        const float2 wm = w[(m - 1) * P];
+        x[m] = DIR < 0 ? cmul(x[m], wm) : cmulconj(x[m], wm);
+      }
+      base[q] = j * P * R + k;
+    } else {
+      base[q] = b * R;
+    }
+    DFT<R, DIR>::run(x);
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[q + Q * m] = x[m];
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int m = 0; m < R; ++m) lds[lds_pad(base[q] + m * P) * es] = v[q + Q * m];
+    __syncthreads();
+  }
+}
+
+template <int E, int DIR, int L, int S = 0>
+__device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__ lds, const int es,
+                                            const int t, const float2* __restrict__ tw) {
+  constexpr CPlanData d = CPlan<L, E>::d;
+  if constexpr (S < d.ns) {
+    constexpr int T = CPlan<L, E>::T;
+    if constexpr (S > 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = lds[lds_pad(t + T * e) * es];
+    }
+    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns>(v, t, tw, lds, es);
+    fft_line_ct<E, DIR, L, S + 1>(v, lds, es, t, tw);
+  }
+}
+
+// Line FFT with a compile-time plan when L != 0, else the runtime plan `pl`.
+template <int E, int DIR, int L>
+__device__ __forceinline__ void fft_run(float2 (&v)[E], float2* __restrict__ lds, const int es,
+                                        const int t, const LinePlan& pl, const float2* __restrict__ tw) {
+  if constexpr (L != 0) {
+    fft_line_ct<E, DIR, L>(v, lds, es, t, tw);
+  } else {
+    fft_line<E, DIR>(v, lds, es, t, pl, tw);
+  }
+}
+
+// Threads per line: a constant when the plan is compile-time.
+template <int E, int L>
+__device__ __forceinline__ int line_threads(const LinePlan& pl) {
+  if constexpr (L != 0) return L / E;
+  else return pl.T;
 }
 
 }  // namespace holo

@@ -77,6 +77,25 @@ bool dispatch_E(int E, F&& f) {
   }
 }
 
+// Lengths with a compile-time plan (the configured hologram sides); others use the runtime
+// plan. f receives (integral_constant<E>, integral_constant<L or 0>).
+template <class F>
+bool dispatch_EL(int E, int L, F&& f) {
+  switch (L) {
+#define HOLO_L(l, e)                                                          \
+  case l:                                                                     \
+    if (E == e) {                                                             \
+      f(std::integral_constant<int, e>{}, std::integral_constant<int, l>{});  \
+      return true;                                                            \
+    }                                                                         \
+    break;
+    HOLO_L(256, 16) HOLO_L(512, 16) HOLO_L(1024, 16) HOLO_L(2048, 16) HOLO_L(768, 12) HOLO_L(1280, 20)
+#undef HOLO_L
+    default: break;
+  }
+  return dispatch_E(E, [&](auto ec) { f(ec, std::integral_constant<int, 0>{}); });
+}
+
 size_t env_size(const char* name, size_t dflt) {
   const char* v = std::getenv(name);
   if (!v || !*v) return dflt;
@@ -204,6 +223,14 @@ int get_plan(holo_context* h, int L, const Plan1D** out) {
     span *= R;
     rem /= R;
   }
+  p.lp.ntw = static_cast<int>(tw.size());
+  {  // the device's compile-time plans must describe the same schedule and table layout
+    const CPlanData c = make_cplan(L, E);
+    bool same = c.ns == p.lp.ns && c.ntw == p.lp.ntw;
+    for (int i = 0; same && i < c.ns; ++i)
+      same = c.R[i] == p.lp.st[i].R && c.P[i] == p.lp.st[i].p && c.off[i] == p.lp.st[i].twoff;
+    if (!same) return fail(HOLO_ERR_SIZE, "internal: plan mismatch for length " + std::to_string(L));
+  }
   if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
   HOLO_HIP(hipMalloc(&p.dtw, tw.size() * sizeof(float2)));
   HOLO_HIP(hipMemcpy(p.dtw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
@@ -223,10 +250,17 @@ int ilog2(int v) {
   return s;
 }
 
-size_t row_lds(const Geom& g) { return static_cast<size_t>(g.rows) * lds_padded_len(g.nx) * sizeof(float2); }
-size_t col_lds(const Geom& g) { return static_cast<size_t>(g.W) * lds_padded_len(g.ny) * sizeof(float2); }
+// dynamic LDS: the lines of the tile, then the plan's twiddle table
+size_t row_lds(const Geom& g) {
+  return (static_cast<size_t>(g.rows) * lds_padded_len(g.nx) + g.px.ntw) * sizeof(float2);
+}
+size_t col_lds(const Geom& g) {
+  return (static_cast<size_t>(g.W) * lds_padded_len(g.ny) + g.py.ntw) * sizeof(float2);
+}
 
-int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey) {
+// row_threads: target threads per row-pass workgroup when the row count is not tuned (512 for
+// the reconstruction's pass B, which loops over slices; 256 elsewhere).
+int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int row_threads = 256) {
   if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
   const Plan1D *px, *py;
   HOLO_TRY(get_plan(h, nx, &px));
@@ -241,23 +275,26 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey) {
   *Ex = px->E;
   *Ey = py->E;
   const size_t colbytes = static_cast<size_t>(lds_padded_len(ny)) * sizeof(float2);
+  const size_t coltw = static_cast<size_t>(py->lp.ntw) * sizeof(float2);
+  const size_t maxl = static_cast<size_t>(h->max_lds);
   int W = h->tune.cols;
   if (W <= 0) {
     W = 16;
-    while (W > 1 && (W * py->lp.T > 512 || W * colbytes > static_cast<size_t>(h->max_lds))) W /= 2;
+    while (W > 1 && (W * py->lp.T > 512 || W * colbytes + coltw > maxl)) W /= 2;
     W = std::min(W, floor_pow2(nx));
   }
   W = floor_pow2(std::max(W, 1));
-  if (W * py->lp.T > 512 || W * colbytes > static_cast<size_t>(h->max_lds))
+  if (W * py->lp.T > 512 || W * colbytes + coltw > maxl)
     return fail(HOLO_ERR_ARG, "column tile too large for a workgroup");
   g->W = W;
   g->wshift = ilog2(W);
   const size_t rowbytes = static_cast<size_t>(lds_padded_len(nx)) * sizeof(float2);
+  const size_t rowtw = static_cast<size_t>(px->lp.ntw) * sizeof(float2);
   int rows = h->tune.rows;
-  if (rows <= 0) rows = std::max(1, 256 / px->lp.T);
+  if (rows <= 0) rows = std::max(1, row_threads / px->lp.T);
   rows = std::min(rows, ny);
-  while (rows > 1 && (rows * px->lp.T > 512 || rows * rowbytes > static_cast<size_t>(h->max_lds))) --rows;
-  if (rows * px->lp.T > 512 || rows * rowbytes > static_cast<size_t>(h->max_lds))
+  while (rows > 1 && (rows * px->lp.T > 512 || rows * rowbytes + rowtw > maxl)) --rows;
+  if (rows * px->lp.T > 512 || rows * rowbytes + rowtw > maxl)
     return fail(HOLO_ERR_ARG, "row tile too large for a workgroup");
   g->rows = rows;
   return HOLO_OK;
@@ -305,15 +342,16 @@ void allow_lds(K kernel, size_t bytes) {
 int launch_rows_fft(const Geom& g, int Ex, int dir, int src, const void* in, size_t in_stride,
                     float2* out, size_t out_stride, float scale, int nbatch, hipStream_t st) {
   const size_t lds = row_lds(g);
-  bool ok = dispatch_E(Ex, [&](auto ec) {
+  bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
     auto launch = [&](auto kern) {
       allow_lds(kern, lds);
       hipLaunchKernelGGL(kern, row_grid(g, nbatch), row_block(g), lds, st, in, in_stride, out, out_stride, scale, g);
     };
-    if (dir < 0 && src == 0) launch(k_rows_fft<E, -1, 0>);
-    else if (dir < 0) launch(k_rows_fft<E, -1, 1>);
-    else launch(k_rows_fft<E, +1, 0>);
+    if (dir < 0 && src == 0) launch(k_rows_fft<E, L, -1, 0>);
+    else if (dir < 0) launch(k_rows_fft<E, L, -1, 1>);
+    else launch(k_rows_fft<E, L, +1, 0>);
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no kernel for this row length");
   return check_launch();
@@ -322,14 +360,15 @@ int launch_rows_fft(const Geom& g, int Ex, int dir, int src, const void* in, siz
 int launch_cols_fft(const Geom& g, int Ey, int dir, const float2* in, size_t in_stride, float2* out,
                     size_t out_stride, float scale, int nbatch, hipStream_t st) {
   const size_t lds = col_lds(g);
-  bool ok = dispatch_E(Ey, [&](auto ec) {
+  bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
     auto launch = [&](auto kern) {
       allow_lds(kern, lds);
       hipLaunchKernelGGL(kern, col_grid(g, nbatch), col_block(g), lds, st, in, in_stride, out, out_stride, scale, g);
     };
-    if (dir < 0) launch(k_cols_fft<E, -1>);
-    else launch(k_cols_fft<E, +1>);
+    if (dir < 0) launch(k_cols_fft<E, L, -1>);
+    else launch(k_cols_fft<E, L, +1>);
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no kernel for this column length");
   return check_launch();
@@ -368,11 +407,13 @@ struct ReconPlan {
 ReconPlan recon_plan(holo_context* h, const Geom& g, int slices) {
   ReconPlan r;
   const size_t plane = static_cast<size_t>(g.nx) * g.ny * sizeof(float2);
-  const size_t budget = env_size("HOLO_BATCH_BYTES", 128ull << 20);
+  const size_t budget = env_size("HOLO_BATCH_BYTES", 256ull << 20);
   int m = h->tune.batch > 0 ? h->tune.batch : static_cast<int>(std::max<size_t>(1, budget / plane));
   r.m = std::max(1, std::min(m, slices));
-  r.spg_c = h->tune.spg_cols > 0 ? h->tune.spg_cols : 2;
-  r.spg_r = h->tune.spg_rows > 0 ? h->tune.spg_rows : 4;
+  // Column workgroups advance up to 16 slices by the ComplexF32 rotation (drift <= 2e-6);
+  // row workgroups merge 8 slices per atomic min.
+  r.spg_c = h->tune.spg_cols > 0 ? h->tune.spg_cols : 16;
+  r.spg_r = h->tune.spg_rows > 0 ? h->tune.spg_rows : 8;
   return r;
 }
 
@@ -392,7 +433,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
   if (!src) return fail(HOLO_ERR_ARG, "input is NULL");
   Geom g;
   int Ex, Ey;
-  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 512));
   const ReconPlan rp = recon_plan(h, g, slices);
   const size_t plane = static_cast<size_t>(nx) * ny;
   const bool arrays = !front->generated;
@@ -430,14 +471,15 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
       ra.first = b0 == 0;
     }
     hipEvent_t pa = prof_begin(h, st);
-    bool ok = dispatch_E(Ey, [&](auto ec) {
+    bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
       constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
       if (!arrays) {
-        allow_lds(k_recon_cols<E, 0>, clds);
-        hipLaunchKernelGGL((k_recon_cols<E, 0>), col_grid(g, (nb + rp.spg_c - 1) / rp.spg_c), col_block(g), clds, st, ra, g);
+        allow_lds(k_recon_cols<E, L, 0>, clds);
+        hipLaunchKernelGGL((k_recon_cols<E, L, 0>), col_grid(g, (nb + rp.spg_c - 1) / rp.spg_c), col_block(g), clds, st, ra, g);
       } else {
-        allow_lds(k_recon_cols<E, 1>, clds);
-        hipLaunchKernelGGL((k_recon_cols<E, 1>), col_grid(g, 1), col_block(g), clds, st, ra, g);
+        allow_lds(k_recon_cols<E, L, 1>, clds);
+        hipLaunchKernelGGL((k_recon_cols<E, L, 1>), col_grid(g, 1), col_block(g), clds, st, ra, g);
       }
     });
     if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
@@ -445,11 +487,12 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     prof_end(h, st, pa, HOLO_PASS_RECON_COLS, nb);
     const int ry = (nb + rp.spg_r - 1) / rp.spg_r;
     hipEvent_t pb = prof_begin(h, st);
-    ok = dispatch_E(Ex, [&](auto ec) {
+    ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
       constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
       if (kind == 0) {
-        allow_lds(k_recon_rows_proj<E>, rlds);
-        hipLaunchKernelGGL(k_recon_rows_proj<E>, row_grid(g, ry), row_block(g), rlds, st, G, plane, nb, rp.spg_r, proj, sc, g);
+        allow_lds(k_recon_rows_proj<E, L>, rlds);
+        hipLaunchKernelGGL((k_recon_rows_proj<E, L>), row_grid(g, ry), row_block(g), rlds, st, G, plane, nb, rp.spg_r, proj, sc, g);
       } else {
         StoreArgs sa{};
         sa.G = G;
@@ -470,11 +513,11 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
           hipLaunchKernelGGL(kern, row_grid(g, ry), row_block(g), rlds, st, sa, g);
         };
         switch (out_dtype) {
-          case HOLO_N0F8: launch(k_recon_rows_store<E, 0>); break;
-          case HOLO_F16: launch(k_recon_rows_store<E, 1>); break;
-          case HOLO_F32: launch(k_recon_rows_store<E, 2>); break;
-          case HOLO_F64: launch(k_recon_rows_store<E, 3>); break;
-          default: launch(k_recon_rows_store<E, 4>); break;
+          case HOLO_N0F8: launch(k_recon_rows_store<E, L, 0>); break;
+          case HOLO_F16: launch(k_recon_rows_store<E, L, 1>); break;
+          case HOLO_F32: launch(k_recon_rows_store<E, L, 2>); break;
+          case HOLO_F64: launch(k_recon_rows_store<E, L, 3>); break;
+          default: launch(k_recon_rows_store<E, L, 4>); break;
         }
       }
     });
@@ -499,14 +542,15 @@ int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, 
     pa.H = static_cast<const float2*>(t->data);
   }
   const size_t lds = col_lds(g);
-  bool ok = dispatch_E(Ey, [&](auto ec) {
+  bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
     if (t->generated) {
-      allow_lds(k_prop_cols<E, 0>, lds);
-      hipLaunchKernelGGL((k_prop_cols<E, 0>), col_grid(g), col_block(g), lds, st, pa, g);
+      allow_lds(k_prop_cols<E, L, 0>, lds);
+      hipLaunchKernelGGL((k_prop_cols<E, L, 0>), col_grid(g), col_block(g), lds, st, pa, g);
     } else {
-      allow_lds(k_prop_cols<E, 1>, lds);
-      hipLaunchKernelGGL((k_prop_cols<E, 1>), col_grid(g), col_block(g), lds, st, pa, g);
+      allow_lds(k_prop_cols<E, L, 1>, lds);
+      hipLaunchKernelGGL((k_prop_cols<E, L, 1>), col_grid(g), col_block(g), lds, st, pa, g);
     }
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
@@ -516,14 +560,15 @@ int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, 
 int launch_gs_rows(const Geom& g, int Ex, int mode, const float2* U, const float* I, float2* out,
                    float sc, hipStream_t st) {
   const size_t lds = row_lds(g);
-  bool ok = dispatch_E(Ex, [&](auto ec) {
+  bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
     if (mode == 0) {
-      allow_lds(k_gs_rows<E, 0>, lds);
-      hipLaunchKernelGGL((k_gs_rows<E, 0>), row_grid(g), row_block(g), lds, st, U, I, out, sc, g);
+      allow_lds(k_gs_rows<E, L, 0>, lds);
+      hipLaunchKernelGGL((k_gs_rows<E, L, 0>), row_grid(g), row_block(g), lds, st, U, I, out, sc, g);
     } else {
-      allow_lds(k_gs_rows<E, 1>, lds);
-      hipLaunchKernelGGL((k_gs_rows<E, 1>), row_grid(g), row_block(g), lds, st, U, I, out, sc, g);
+      allow_lds(k_gs_rows<E, L, 1>, lds);
+      hipLaunchKernelGGL((k_gs_rows<E, L, 1>), row_grid(g), row_block(g), lds, st, U, I, out, sc, g);
     }
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");

@@ -115,32 +115,44 @@ __global__ void k_gabor(size_t n, const float* __restrict__ holo, float2* __rest
 // ------------------------------------------------------------------------------------------
 // helpers shared by the pass kernels
 // ------------------------------------------------------------------------------------------
+// LDS of a row pass: `rows` padded lines, then the x plan's twiddle table.
 struct RowCtx {
   int t, w, j;
   bool ok;
   float2* line;
+  const float2* tw;
 };
+template <int E, int L>
 __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
   RowCtx c;
-  const int T = g.px.T;
+  const int T = line_threads<E, L>(g.px);
   c.t = threadIdx.x % T;
   c.w = threadIdx.x / T;
   c.j = blockIdx.x * g.rows + c.w;
   c.ok = c.j < g.ny;
   c.line = lds + c.w * lds_padded_len(g.nx);
+  float2* tw = lds + g.rows * lds_padded_len(g.nx);
+  stage_twiddles(tw, g.twx, g.px.ntw);
+  c.tw = tw;
   return c;
 }
 
+// LDS of a column pass: W interleaved padded columns, then the y plan's twiddle table.
 struct ColCtx {
   int t, w, i;
   float2* col;
+  const float2* tw;
 };
+template <int E, int L>
 __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   ColCtx c;
   c.w = threadIdx.x & (g.W - 1);
   c.t = threadIdx.x >> g.wshift;
   c.i = blockIdx.x * g.W + c.w;
   c.col = lds + c.w;
+  float2* tw = lds + g.W * lds_padded_len(g.ny);
+  stage_twiddles(tw, g.twy, g.py.ntw);
+  c.tw = tw;
   return c;
 }
 
@@ -150,13 +162,13 @@ __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
 // Row pass with an optional scale on the output. SRC 0: complex input; SRC 1: real
 // intensity I, input sqrt(I) (the Gabor wavefront, holofunc.jl:81, fused into the first FFT
 // pass).
-template <int E, int DIR, int SRC>
+template <int E, int L, int DIR, int SRC>
 __global__ __launch_bounds__(512) void k_rows_fft(const void* __restrict__ in, size_t in_stride,
                                                   float2* __restrict__ out, size_t out_stride,
                                                   float scale, Geom g) {
   extern __shared__ float2 lds[];
-  const RowCtx c = row_ctx(g, lds);
-  const int T = g.px.T;
+  const RowCtx c = row_ctx<E, L>(g, lds);
+  const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
   const size_t zin = blockIdx.y * in_stride, zout = blockIdx.y * out_stride;
   float2 v[E];
@@ -171,7 +183,7 @@ __global__ __launch_bounds__(512) void k_rows_fft(const void* __restrict__ in, s
       v[e] = make_float2(sqrtf(static_cast<const float*>(in)[zin + rowoff + i]), 0.f);
     }
   }
-  fft_line<E, DIR>(v, c.line, 1, c.t, g.px, g.twx);
+  fft_run<E, DIR, L>(v, c.line, 1, c.t, g.px, c.tw);
   if (c.ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e) out[zout + rowoff + c.t + T * e] = cscale(v[e], scale);
@@ -179,20 +191,20 @@ __global__ __launch_bounds__(512) void k_rows_fft(const void* __restrict__ in, s
 }
 
 // Column pass with an optional scale on the output.
-template <int E, int DIR>
+template <int E, int L, int DIR>
 __global__ __launch_bounds__(512) void k_cols_fft(const float2* __restrict__ in, size_t in_stride,
                                                   float2* __restrict__ out, size_t out_stride,
                                                   float scale, Geom g) {
   extern __shared__ float2 lds[];
-  const ColCtx c = col_ctx(g, lds);
-  const int T = g.py.T;
+  const ColCtx c = col_ctx<E, L>(g, lds);
+  const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const size_t zin = blockIdx.y * in_stride, zout = blockIdx.y * out_stride;
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e)
     v[e] = ok ? in[zin + static_cast<size_t>(c.t + T * e) * g.nx + c.i] : make_float2(0.f, 0.f);
-  fft_line<E, DIR>(v, c.col, g.W, c.t, g.py, g.twy);
+  fft_run<E, DIR, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if (ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -203,8 +215,10 @@ __global__ __launch_bounds__(512) void k_cols_fft(const float2* __restrict__ in,
 // ------------------------------------------------------------------------------------------
 // Reconstruction pass A (column inverse FFT of F .* H_k), A6-A9
 // ------------------------------------------------------------------------------------------
-// HMODE 0 (generated): H_k = exp(2 pi i frac((zl0 + k*dzl) * sqrt(d))) in FP64-wrapped form.
-//   The WG keeps F and sqrt(d) (FP64) for its W x ny tile in registers across its slices.
+// HMODE 0 (generated): H_k = exp(2 pi i frac((zl0 + k*dzl) * sqrt(d))). A workgroup owns
+//   `spg` consecutive slices of a W x ny tile: it evaluates H at its first slice and the
+//   per-slice rotation exp(2 pi i frac(dzl * sqrt(d))) from FP64-wrapped phases, then advances
+//   F .* H_k by one ComplexF32 multiply per slice (relative drift <= spg * 1.2e-7).
 // HMODE 1 (arrays, the reference's running product, holofunc.jl:254-260): the ComplexF32
 //   running spectrum R_k = R_{k-1} .* Hdz (R_0 = F .* H0) is advanced in registers; the WG
 //   covers every slice of the batch in order and stores R back to `run` at the end.
@@ -228,11 +242,11 @@ __device__ __forceinline__ size_t centred_index(int i, int j, int nx, int ny) {
   return static_cast<size_t>(jc) * nx + ic;
 }
 
-template <int E, int HMODE>
+template <int E, int L, int HMODE>
 __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, Geom g) {
   extern __shared__ float2 lds[];
-  const ColCtx c = col_ctx(g, lds);
-  const int T = g.py.T;
+  const ColCtx c = col_ctx<E, L>(g, lds);
+  const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
   float2 f[E];
@@ -240,27 +254,37 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, Geom g) {
   for (int e = 0; e < E; ++e) f[e] = ok ? a.F[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
 
   if constexpr (HMODE == 0) {
-    double s[E];
+    // f <- F .* H_{k0} and r = exp(2 pi i dzl sqrt(d)), both from FP64-wrapped phases; then
+    // every further slice of the group advances f <- f .* r (one complex multiply).
+    const int k0 = blockIdx.y * a.spg;
+    const int k1 = min(k0 + a.spg, a.nbatch);
+    const double zl = a.gp.zl0 + k0 * a.gp.dzl;
+    float2 r[E];
     {
       const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
 #pragma clang fp contract(off)
         const double d = (1.0 - ax) - axis_a(c.t + T * e, g.ny, a.gp.wavlen, a.gp.dx);
-        s[e] = d >= 0.0 ? sqrt(d) : -1.0;  // -1 marks an evanescent component
+        if (d >= 0.0) {
+          const double sq = sqrt(d);
+          f[e] = cmul(f[e], cis_cycles(zl * sq));
+          r[e] = cis_cycles(a.gp.dzl * sq);
+        } else {  // evanescent component: H = 0
+          f[e] = make_float2(0.f, 0.f);
+          r[e] = make_float2(0.f, 0.f);
+        }
       }
     }
-    const int k0 = blockIdx.y * a.spg;
-    const int k1 = min(k0 + a.spg, a.nbatch);
     for (int k = k0; k < k1; ++k) {
-      const double zl = a.gp.zl0 + k * a.gp.dzl;
       float2 v[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const float2 h = s[e] >= 0.0 ? cis_cycles(zl * s[e]) : make_float2(0.f, 0.f);
-        v[e] = cmul(f[e], h);
+      for (int e = 0; e < E; ++e) v[e] = f[e];
+      if (k + 1 < k1) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
       }
-      fft_line<E, +1>(v, c.col, g.W, c.t, g.py, g.twy);
+      fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
       if (ok) {
         float2* __restrict__ dst = a.G + static_cast<size_t>(k) * a.gstride;
 #pragma unroll
@@ -287,7 +311,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, Geom g) {
       float2 v[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = f[e];
-      fft_line<E, +1>(v, c.col, g.W, c.t, g.py, g.twy);
+      fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
       if (ok) {
         float2* __restrict__ dst = a.G + static_cast<size_t>(k) * a.gstride;
 #pragma unroll
@@ -320,14 +344,14 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, size_t n, unsigned int 
 
 // Projection epilogue (A6): proj = min over slices of |psi_k|^2, psi_k = IFFT/(nx*ny). The
 // minimum follows CUDA.jl's device min for Float32 (fminf: a NaN operand is ignored).
-template <int E>
+template <int E, int L>
 __global__ __launch_bounds__(512) void k_recon_rows_proj(const float2* __restrict__ G,
                                                          size_t gstride, int nbatch, int spg,
                                                          float* __restrict__ proj, float sc,
                                                          Geom g) {
   extern __shared__ float2 lds[];
-  const RowCtx c = row_ctx(g, lds);
-  const int T = g.px.T;
+  const RowCtx c = row_ctx<E, L>(g, lds);
+  const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
   const int k0 = blockIdx.y * spg;
   const int k1 = min(k0 + spg, nbatch);
@@ -339,7 +363,7 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(const float2* __restric
     const float2* __restrict__ src = G + static_cast<size_t>(k) * gstride + rowoff;
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = c.ok ? src[c.t + T * e] : make_float2(0.f, 0.f);
-    fft_line<E, +1>(v, c.line, 1, c.t, g.px, g.twx);
+    fft_run<E, +1, L>(v, c.line, 1, c.t, g.px, c.tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float a = v[e].x * sc, b = v[e].y * sc;
@@ -397,11 +421,11 @@ __device__ __forceinline__ float store_value(void* vol, size_t idx, float2 psi) 
   }
 }
 
-template <int E, int OUT>
+template <int E, int L, int OUT>
 __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
   extern __shared__ float2 lds[];
-  const RowCtx c = row_ctx(g, lds);
-  const int T = g.px.T;
+  const RowCtx c = row_ctx<E, L>(g, lds);
+  const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
   const int oj = c.j - a.oy;
   const bool rowin = c.ok && oj >= 0 && oj < a.ony;
@@ -415,7 +439,7 @@ __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
     const float2* __restrict__ src = a.G + static_cast<size_t>(k) * a.gstride + rowoff;
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = c.ok ? src[c.t + T * e] : make_float2(0.f, 0.f);
-    fft_line<E, +1>(v, c.line, 1, c.t, g.px, g.twx);
+    fft_run<E, +1, L>(v, c.line, 1, c.t, g.px, c.tw);
     if (rowin) {
       const size_t sbase = static_cast<size_t>(a.slice0 + k) * a.vstride + static_cast<size_t>(oj) * a.onx;
 #pragma unroll
@@ -445,18 +469,18 @@ struct PropArgs {
   const float2* H;     // HMODE 1
 };
 
-template <int E, int HMODE>
+template <int E, int L, int HMODE>
 __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, Geom g) {
   extern __shared__ float2 lds[];
-  const ColCtx c = col_ctx(g, lds);
-  const int T = g.py.T;
+  const ColCtx c = col_ctx<E, L>(g, lds);
+  const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e)
     v[e] = ok ? a.in[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
-  fft_line<E, -1>(v, c.col, g.W, c.t, g.py, g.twy);
+  fft_run<E, -1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if constexpr (HMODE == 0) {
     const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
 #pragma unroll
@@ -470,7 +494,7 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, Geom g) {
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = cmul(v[e], a.H[centred_index(ci, c.t + T * e, g.nx, g.ny)]);
   }
-  fft_line<E, +1>(v, c.col, g.W, c.t, g.py, g.twy);
+  fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if (ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e) a.out[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
@@ -481,18 +505,18 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, Geom g) {
 // replacement psi <- sqrt(I) * psi/|psi| (|psi| = 0 -> sqrt(I), i.e. angle(0) = 0), then
 // MODE 0: forward row FFT of the result into `out` (first half of the next propagation);
 // MODE 1: store the result (the retrieved wavefront).
-template <int E, int MODE>
+template <int E, int L, int MODE>
 __global__ __launch_bounds__(512) void k_gs_rows(const float2* __restrict__ U,
                                                  const float* __restrict__ I,
                                                  float2* __restrict__ out, float sc, Geom g) {
   extern __shared__ float2 lds[];
-  const RowCtx c = row_ctx(g, lds);
-  const int T = g.px.T;
+  const RowCtx c = row_ctx<E, L>(g, lds);
+  const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = c.ok ? U[rowoff + c.t + T * e] : make_float2(0.f, 0.f);
-  fft_line<E, +1>(v, c.line, 1, c.t, g.px, g.twx);
+  fft_run<E, +1, L>(v, c.line, 1, c.t, g.px, c.tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const float A = c.ok ? sqrtf(I[rowoff + c.t + T * e]) : 0.f;
@@ -505,7 +529,7 @@ __global__ __launch_bounds__(512) void k_gs_rows(const float2* __restrict__ U,
       v[e] = make_float2(A, 0.f);
     }
   }
-  if constexpr (MODE == 0) fft_line<E, -1>(v, c.line, 1, c.t, g.px, g.twx);
+  if constexpr (MODE == 0) fft_run<E, -1, L>(v, c.line, 1, c.t, g.px, c.tw);
   if (c.ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e) out[rowoff + c.t + T * e] = v[e];

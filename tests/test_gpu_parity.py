@@ -98,8 +98,10 @@ def test_projection_generated_vs_fp64_oracle(gpu, n, slices, z0, dz):
     got = ph.cu_get_reconst_xyprojection(wf, h0, hdz, slices).cpu().numpy()
     want = asm.reconst_xyprojection(asm.gabor_wavefront(h), slices, "exact", z_front=z0, z_dz=dz, wavlen=LAM, dx=DX)
     assert asm.rel_l2(got, want) < 1e-4
+    # sqrt(I) fused into the first FFT pass: same arithmetic up to FMA contraction of the
+    # zero imaginary part
     fused = ph.reconst_xyprojection_holo(_t(h, gpu), h0, hdz, slices).cpu().numpy()
-    np.testing.assert_array_equal(fused, got)
+    assert asm.rel_l2(fused, got) < 1e-6
 
 
 @pytest.mark.parametrize("batch,spgc,spgr", [(1, 1, 1), (3, 2, 1), (4, 3, 5), (64, 7, 2)])
