@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the plain-path and GS figures")
+    ap.add_argument("--profile-gs", action="store_true",
+                    help="for rocprofv3: Gerchberg-Saxton at 1024^2 (one warm-up call, then --steps "
+                         "calls of 20 iterations: eager launches, then the hipGraph form); no extras")
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3: one warm-up + --steps steps, then a 256 MiB device copy "
                          "(a kernel of known traffic to scale FETCH_SIZE / WRITE_SIZE); no extras")
@@ -85,6 +88,18 @@ def main():
 
     def step():
         ph.reconst_xyprojection_holo(holo, h0, hdz, SLICES, out=proj)
+
+    if args.profile_gs:
+        i2 = torch.from_numpy(synth.gs_pair(103, 500, N, 2000.0)[1]).to(dev)
+        hf, hb = ph.cu_transfer(2000.0, N, LAM, d_sqr), ph.cu_transfer(-2000.0, N, LAM, d_sqr)
+        out = None
+        for use_graph in (False, True):
+            for _ in range(1 + args.steps):
+                out = ph.cu_phase_retrieval_holo(holo, i2, hf, hb, 20, N, use_graph=use_graph)
+            torch.cuda.synchronize()
+        log("bench: GS profile run done")
+        print(json.dumps({"profile_gs": True, "steps": args.steps, "finite": bool(torch.isfinite(out.data).all())}), flush=True)
+        return
 
     if args.profile_run:
         timed_steps(step, args.steps, 1, torch)

@@ -147,6 +147,32 @@ int holo_asm_prop(holo_handle h, void* stream, int nx, int ny, void* out, const 
 int holo_fft2(holo_handle h, void* stream, int nx, int ny, const void* in, void* out,
               int inverse);
 
+/* ---- batched reconstruction (no counterpart in the reference: its functions take one
+ *      hologram; this runs many cu_get_reconst_xyprojection calls, holofunc.jl:248-266, together) */
+typedef struct holo_job {
+  int nx, ny;
+  const void* src;   /* Float32 hologram (src_real = 1, wavefront sqrt(holo)) or ComplexF32 wavefront */
+  int src_real;
+  holo_tf front;     /* generated transfers only */
+  holo_tf dz;
+  int slices;
+  float* proj;       /* Float32 nx*ny min-projection */
+} holo_job;
+/* Reconstructs every job's xy min-projection. Jobs are grouped by (nx, ny, wavlen, dx); one
+ * column-pass launch and one row-pass launch per round cover every hologram of a group (up to
+ * 32 at a time; workspace of (1 + batch) planes per hologram, chunked to HOLO_BATCH_WS_BYTES).
+ * Each result equals the single-hologram call's. */
+int holo_reconst_batch(holo_handle h, void* stream, const holo_job* jobs, int njobs);
+/* Per-size tile choices (0 = default): slices per batch, slices per column / row workgroup,
+ * columns per workgroup, rows per workgroup. The global holo_set_tuning overrides them. */
+int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_cols, int spg_rows,
+                             int cols_per_wg, int rows_per_wg);
+/* The effective choices for a size (defaults resolved) as {batch, spg_cols, spg_rows, cols, rows}. */
+int holo_get_tuning_for_size(holo_handle h, int nx, int ny, int slices, int* out5);
+/* Times candidate tile configurations of both reconstruction passes on the device (HIP events,
+ * synchronous) and keeps the fastest for this size; out5 receives it (may be NULL). */
+int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, double dx, int* out5);
+
 /* ---- measurement (no counterpart in the reference, which has no timers) ---------------- */
 /* Passes timed with HIP events when profiling is on: the fused reconstruction's column pass
  * (pass A) and row pass (pass B), the forward spectrum, the GS propagation column pass and

@@ -38,11 +38,24 @@ struct LinePlan {
   StagePlan st[kMaxStages];
 };
 
-// Copies a plan's twiddle table into LDS (every thread of the workgroup participates). The
-// first stage needs no twiddles and ends with a barrier before its LDS writes, so the copy
-// is complete and visible before any stage reads it.
+// Copies a plan's twiddle table into LDS (every thread of the workgroup participates). Kernels
+// call it after issuing their first data loads; up to 8 loads per thread are in flight
+// together. The first FFT stage needs no twiddles and ends with a barrier before its LDS
+// writes, so the copy is complete and visible before any stage reads it.
 __device__ __forceinline__ void stage_twiddles(float2* __restrict__ dst, const float2* __restrict__ src, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+  const int nt = blockDim.x;
+  float2 tmp[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = threadIdx.x + k * nt;
+    if (i < n) tmp[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = threadIdx.x + k * nt;
+    if (i < n) dst[i] = tmp[k];
+  }
+  for (int i = threadIdx.x + 8 * nt; i < n; i += nt) dst[i] = src[i];
 }
 
 // ------------------------------------------------------------------------------------------

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <tuple>
@@ -110,7 +111,8 @@ struct holo_context {
   std::map<int, Plan1D> plans;
   void* ws = nullptr;
   size_t ws_bytes = 0;
-  Tuning tune;
+  Tuning tune;                                    // global override (tests, sweeps)
+  std::map<std::pair<int, int>, Tuning> size_tune;  // per-size choices (autotuner)
   hipStream_t cap = nullptr;
   double2* partials = nullptr;  // 256 partial sums for cu_2d_pad's mean
   struct GsKey {
@@ -258,6 +260,20 @@ size_t col_lds(const Geom& g) {
   return (static_cast<size_t>(g.W) * lds_padded_len(g.ny) + g.py.ntw) * sizeof(float2);
 }
 
+// Effective tuning for a size: the per-size entry, with every nonzero field of the global
+// override taking precedence.
+Tuning eff_tuning(const holo_context* h, int nx, int ny) {
+  Tuning t;
+  auto it = h->size_tune.find({nx, ny});
+  if (it != h->size_tune.end()) t = it->second;
+  if (h->tune.batch > 0) t.batch = h->tune.batch;
+  if (h->tune.spg_cols > 0) t.spg_cols = h->tune.spg_cols;
+  if (h->tune.spg_rows > 0) t.spg_rows = h->tune.spg_rows;
+  if (h->tune.cols > 0) t.cols = h->tune.cols;
+  if (h->tune.rows > 0) t.rows = h->tune.rows;
+  return t;
+}
+
 // row_threads: target threads per row-pass workgroup when the row count is not tuned (512 for
 // the reconstruction's pass B, which loops over slices; 256 elsewhere).
 int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int row_threads = 256) {
@@ -277,7 +293,8 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
   const size_t colbytes = static_cast<size_t>(lds_padded_len(ny)) * sizeof(float2);
   const size_t coltw = static_cast<size_t>(py->lp.ntw) * sizeof(float2);
   const size_t maxl = static_cast<size_t>(h->max_lds);
-  int W = h->tune.cols;
+  const Tuning tn = eff_tuning(h, nx, ny);
+  int W = tn.cols;
   if (W <= 0) {
     W = 16;
     while (W > 1 && (W * py->lp.T > 512 || W * colbytes + coltw > maxl)) W /= 2;
@@ -290,7 +307,7 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
   g->wshift = ilog2(W);
   const size_t rowbytes = static_cast<size_t>(lds_padded_len(nx)) * sizeof(float2);
   const size_t rowtw = static_cast<size_t>(px->lp.ntw) * sizeof(float2);
-  int rows = h->tune.rows;
+  int rows = tn.rows;
   if (rows <= 0) rows = std::max(1, row_threads / px->lp.T);
   rows = std::min(rows, ny);
   while (rows > 1 && (rows * px->lp.T > 512 || rows * rowbytes + rowtw > maxl)) --rows;
@@ -408,12 +425,13 @@ ReconPlan recon_plan(holo_context* h, const Geom& g, int slices) {
   ReconPlan r;
   const size_t plane = static_cast<size_t>(g.nx) * g.ny * sizeof(float2);
   const size_t budget = env_size("HOLO_BATCH_BYTES", 256ull << 20);
-  int m = h->tune.batch > 0 ? h->tune.batch : static_cast<int>(std::max<size_t>(1, budget / plane));
+  const Tuning tn = eff_tuning(h, g.nx, g.ny);
+  int m = tn.batch > 0 ? tn.batch : static_cast<int>(std::max<size_t>(1, budget / plane));
   r.m = std::max(1, std::min(m, slices));
   // Column workgroups advance up to 16 slices by the ComplexF32 rotation (drift <= 2e-6);
   // row workgroups merge 8 slices per atomic min.
-  r.spg_c = h->tune.spg_cols > 0 ? h->tune.spg_cols : 16;
-  r.spg_r = h->tune.spg_rows > 0 ? h->tune.spg_rows : 8;
+  r.spg_c = tn.spg_cols > 0 ? tn.spg_cols : 16;
+  r.spg_r = tn.spg_rows > 0 ? tn.spg_rows : 8;
   return r;
 }
 
@@ -471,15 +489,23 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
       ra.first = b0 == 0;
     }
     hipEvent_t pa = prof_begin(h, st);
+    JobTable jt{};
+    jt.n = 1;
+    jt.d[0].F = F;
+    jt.d[0].G = G;
+    jt.d[0].proj = proj;
+    jt.d[0].zl0 = ra.gp.zl0;
+    jt.d[0].dzl = ra.gp.dzl;
+    jt.d[0].nbatch = nb;
     bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
       constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
       if (!arrays) {
         allow_lds(k_recon_cols<E, L, 0>, clds);
-        hipLaunchKernelGGL((k_recon_cols<E, L, 0>), col_grid(g, (nb + rp.spg_c - 1) / rp.spg_c), col_block(g), clds, st, ra, g);
+        hipLaunchKernelGGL((k_recon_cols<E, L, 0>), col_grid(g, (nb + rp.spg_c - 1) / rp.spg_c), col_block(g), clds, st, ra, jt, g);
       } else {
         allow_lds(k_recon_cols<E, L, 1>, clds);
-        hipLaunchKernelGGL((k_recon_cols<E, L, 1>), col_grid(g, 1), col_block(g), clds, st, ra, g);
+        hipLaunchKernelGGL((k_recon_cols<E, L, 1>), col_grid(g, 1), col_block(g), clds, st, ra, jt, g);
       }
     });
     if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
@@ -492,7 +518,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     constexpr int L = decltype(lc)::value;
       if (kind == 0) {
         allow_lds(k_recon_rows_proj<E, L>, rlds);
-        hipLaunchKernelGGL((k_recon_rows_proj<E, L>), row_grid(g, ry), row_block(g), rlds, st, G, plane, nb, rp.spg_r, proj, sc, g);
+        hipLaunchKernelGGL((k_recon_rows_proj<E, L>), row_grid(g, ry), row_block(g), rlds, st, jt, plane, rp.spg_r, sc, g);
       } else {
         StoreArgs sa{};
         sa.G = G;
@@ -528,13 +554,66 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
   return HOLO_OK;
 }
 
-// One propagation column pass: in -> FFT_col -> .* H -> IFFT_col -> out
+// One round of the generated-H reconstruction for every hologram in `jt` (one size): pass A
+// over (column tiles, slice groups, holograms), then pass B over (row tiles, slice groups,
+// holograms). `maxnb` is the largest per-hologram slice count of the round.
+int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan& rp, double wavlen,
+                 double dx, const JobTable& jt, int maxnb, hipStream_t st, bool do_a = true,
+                 bool do_b = true) {
+  const size_t plane = static_cast<size_t>(g.nx) * g.ny;
+  const float sc = static_cast<float>(1.0 / (static_cast<double>(g.nx) * g.ny));
+  long long slices = 0;
+  for (int i = 0; i < jt.n; ++i) slices += jt.d[i].nbatch;
+  if (do_a) {
+    RecArgs ra{};
+    ra.gstride = plane;
+    ra.spg = rp.spg_c;
+    ra.gp.wavlen = wavlen;
+    ra.gp.dx = dx;
+    const size_t clds = col_lds(g);
+    hipEvent_t pa = prof_begin(h, st);
+    bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
+      constexpr int E = decltype(ec)::value;
+      constexpr int L = decltype(lc)::value;
+      allow_lds(k_recon_cols<E, L, 0>, clds);
+      dim3 grid = col_grid(g, (maxnb + rp.spg_c - 1) / rp.spg_c);
+      grid.z = jt.n;
+      hipLaunchKernelGGL((k_recon_cols<E, L, 0>), grid, col_block(g), clds, st, ra, jt, g);
+    });
+    if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
+    HOLO_TRY(check_launch());
+    prof_end(h, st, pa, HOLO_PASS_RECON_COLS, slices);
+  }
+  if (do_b) {
+    const size_t rlds = row_lds(g);
+    hipEvent_t pb = prof_begin(h, st);
+    bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
+      constexpr int E = decltype(ec)::value;
+      constexpr int L = decltype(lc)::value;
+      allow_lds(k_recon_rows_proj<E, L>, rlds);
+      dim3 grid = row_grid(g, (maxnb + rp.spg_r - 1) / rp.spg_r);
+      grid.z = jt.n;
+      hipLaunchKernelGGL((k_recon_rows_proj<E, L>), grid, row_block(g), rlds, st, jt, plane, rp.spg_r, sc, g);
+    });
+    if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
+    HOLO_TRY(check_launch());
+    prof_end(h, st, pb, HOLO_PASS_RECON_ROWS, slices);
+  }
+  return HOLO_OK;
+}
+
+// One propagation column pass: in -> FFT_col -> .* H -> IFFT_col -> out. With Hnat != NULL
+// the natural-order array is used (conjugated when conj != 0) instead of `t`.
 int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, float2* out,
-                     hipStream_t st) {
+                     hipStream_t st, const float2* Hnat = nullptr, int conj = 0) {
   PropArgs pa{};
   pa.in = in;
   pa.out = out;
-  if (t->generated) {
+  const int mode = Hnat ? 2 : (t->generated ? 0 : 1);
+  if (mode == 2) {
+    pa.H = Hnat;
+    pa.conj = conj;
+  } else if (mode == 0) {
     pa.gp.wavlen = t->wavlen;
     pa.gp.dx = t->dx;
     pa.gp.zl0 = t->z / t->wavlen;
@@ -545,13 +624,13 @@ int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, 
   bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
-    if (t->generated) {
-      allow_lds(k_prop_cols<E, L, 0>, lds);
-      hipLaunchKernelGGL((k_prop_cols<E, L, 0>), col_grid(g), col_block(g), lds, st, pa, g);
-    } else {
-      allow_lds(k_prop_cols<E, L, 1>, lds);
-      hipLaunchKernelGGL((k_prop_cols<E, L, 1>), col_grid(g), col_block(g), lds, st, pa, g);
-    }
+    auto launch = [&](auto kern) {
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, col_grid(g), col_block(g), lds, st, pa, g);
+    };
+    if (mode == 0) launch(k_prop_cols<E, L, 0>);
+    else if (mode == 1) launch(k_prop_cols<E, L, 1>);
+    else launch(k_prop_cols<E, L, 2>);
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
   return check_launch();
@@ -575,12 +654,15 @@ int launch_gs_rows(const Geom& g, int Ex, int mode, const float2* U, const float
   return check_launch();
 }
 
+// One GS iteration. hf / hb: natural-order transfer arrays (hb may equal hf with conj_b set),
+// or NULL to use the transfers as given.
 int gs_iteration(holo_context* h, const Geom& g, int Ex, int Ey, float2* B, const float* I1,
-                 const float* I2, const holo_tf* fwd, const holo_tf* bwd, float sc, bool last,
-                 float2* out, hipStream_t st) {
+                 const float* I2, const holo_tf* fwd, const holo_tf* bwd, const float2* hf,
+                 const float2* hb, int conj_b, float sc, bool last, float2* out, hipStream_t st) {
   for (int half = 0; half < 2; ++half) {
     hipEvent_t pc = prof_begin(h, st);
-    HOLO_TRY(launch_prop_cols(g, Ey, half == 0 ? fwd : bwd, B, B, st));
+    HOLO_TRY(launch_prop_cols(g, Ey, half == 0 ? fwd : bwd, B, B, st, half == 0 ? hf : hb,
+                              half == 0 ? 0 : conj_b));
     prof_end(h, st, pc, HOLO_PASS_PROP_COLS, 1);
     hipEvent_t pr = prof_begin(h, st);
     if (half == 0) HOLO_TRY(launch_gs_rows(g, Ex, 0, B, I2, B, sc, st));
@@ -807,8 +889,28 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
   int Ex, Ey;
   HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
   const size_t plane = static_cast<size_t>(nx) * ny;
-  HOLO_TRY(ensure_ws(h, plane * sizeof(float2)));
+  // workspace: [field B | H(fwd) | H(bwd)] -- generated transfers are evaluated once per call
+  // in natural order; H(-z) = conj(H(z)) lets a mirrored pair share one array.
+  const bool gen = fwd->generated && bwd->generated;
+  const bool mirrored = gen && bwd->z == -fwd->z && bwd->wavlen == fwd->wavlen && bwd->dx == fwd->dx;
+  const int narr = gen ? (mirrored ? 1 : 2) : 0;
+  HOLO_TRY(ensure_ws(h, plane * sizeof(float2) * (1 + narr)));
   float2* B = static_cast<float2*>(h->ws);
+  float2* hf = nullptr;
+  float2* hb = nullptr;
+  int conj_b = 0;
+  if (gen) {
+    hf = B + plane;
+    hb = mirrored ? hf : hf + plane;
+    conj_b = mirrored ? 1 : 0;
+    const dim3 grid((nx + 255) / 256, ny), block(256);
+    hipLaunchKernelGGL(k_transfer_natural, grid, block, 0, st, nx, ny, fwd->z / fwd->wavlen, fwd->wavlen, fwd->dx, hf);
+    HOLO_TRY(check_launch());
+    if (!mirrored) {
+      hipLaunchKernelGGL(k_transfer_natural, grid, block, 0, st, nx, ny, bwd->z / bwd->wavlen, bwd->wavlen, bwd->dx, hb);
+      HOLO_TRY(check_launch());
+    }
+  }
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
   float2* o = static_cast<float2*>(out);
   HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, holo1, 0, B, 0, 1.0f, 1, st));  // psi1 = sqrt(I1), row FFT
@@ -823,7 +925,7 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
     } else {
       hipGraph_t graph = nullptr;
       HOLO_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
-      const int r = gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, false, o, h->cap);
+      const int r = gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, hf, hb, conj_b, sc, false, o, h->cap);
       const hipError_t e = hipStreamEndCapture(h->cap, &graph);
       if (r != HOLO_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -838,9 +940,9 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
     for (int it2 = 0; it2 < body; ++it2) HOLO_HIP(hipGraphLaunch(exec, st));
   } else {
     for (int it2 = 0; it2 < body; ++it2)
-      HOLO_TRY(gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, false, o, st));
+      HOLO_TRY(gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, hf, hb, conj_b, sc, false, o, st));
   }
-  return gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, sc, true, o, st);
+  return gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, hf, hb, conj_b, sc, true, o, st);
 }
 
 int holo_set_profiling(holo_handle h, int enable) {
@@ -860,6 +962,198 @@ int holo_profile_read(holo_handle h, int pass, double* total_ms, long long* laun
     h->prof_ms[pass] = 0.0;
     h->prof_launch[pass] = 0;
     h->prof_units[pass] = 0;
+  }
+  return HOLO_OK;
+}
+
+int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_cols, int spg_rows,
+                             int cols_per_wg, int rows_per_wg) {
+  if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
+  if (cols_per_wg > 0 && (cols_per_wg & (cols_per_wg - 1)) != 0)
+    return fail(HOLO_ERR_ARG, "cols_per_wg must be a power of two");
+  Tuning t;
+  t.batch = batch;
+  t.spg_cols = spg_cols;
+  t.spg_rows = spg_rows;
+  t.cols = cols_per_wg;
+  t.rows = rows_per_wg;
+  h->size_tune[{nx, ny}] = t;
+  return HOLO_OK;
+}
+
+int holo_get_tuning_for_size(holo_handle h, int nx, int ny, int slices, int* out5) {
+  if (!h || !out5) return fail(HOLO_ERR_ARG, "NULL argument");
+  Geom g;
+  int Ex, Ey;
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 512));
+  const ReconPlan rp = recon_plan(h, g, slices > 0 ? slices : 1 << 20);
+  out5[0] = rp.m;
+  out5[1] = rp.spg_c;
+  out5[2] = rp.spg_r;
+  out5[3] = g.W;
+  out5[4] = g.rows;
+  return HOLO_OK;
+}
+
+int holo_reconst_batch(holo_handle h, void* stream, const holo_job* jobs, int njobs) {
+  if (!h || (njobs > 0 && !jobs)) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (njobs < 0) return fail(HOLO_ERR_ARG, "njobs must be >= 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (int i = 0; i < njobs; ++i) {
+    const holo_job& j = jobs[i];
+    if (!j.src || !j.proj) return fail(HOLO_ERR_ARG, "job " + std::to_string(i) + ": NULL input or projection");
+    if (j.slices < 1) return fail(HOLO_ERR_ARG, "job " + std::to_string(i) + ": slices must be >= 1");
+    if (!j.front.generated || !j.dz.generated)
+      return fail(HOLO_ERR_ARG, "job " + std::to_string(i) + ": batched reconstruction needs generated transfers");
+    HOLO_TRY(validate_transfer(&j.front, "front"));
+    HOLO_TRY(validate_transfer(&j.dz, "dz"));
+    if (j.front.wavlen != j.dz.wavlen || j.front.dx != j.dz.dx)
+      return fail(HOLO_ERR_ARG, "job " + std::to_string(i) + ": transfers must share wavlen and dx");
+  }
+  // group by (nx, ny, wavlen, dx), keeping submission order inside a group
+  std::vector<int> order(njobs);
+  for (int i = 0; i < njobs; ++i) order[i] = i;
+  auto key = [&](int i) {
+    return std::make_tuple(jobs[i].nx, jobs[i].ny, jobs[i].front.wavlen, jobs[i].front.dx);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  const size_t budget = env_size("HOLO_BATCH_WS_BYTES", 8ull << 30);
+  for (size_t g0 = 0; g0 < order.size();) {
+    size_t g1 = g0 + 1;
+    while (g1 < order.size() && key(order[g1]) == key(order[g0])) ++g1;
+    const holo_job& j0 = jobs[order[g0]];
+    Geom g;
+    int Ex, Ey;
+    HOLO_TRY(make_geom(h, j0.nx, j0.ny, &g, &Ex, &Ey, 512));
+    int maxs = 0;
+    for (size_t q = g0; q < g1; ++q) maxs = std::max(maxs, jobs[order[q]].slices);
+    const ReconPlan rp = recon_plan(h, g, maxs);
+    const size_t plane = static_cast<size_t>(g.nx) * g.ny;
+    const size_t perjob = plane * sizeof(float2) * (1 + static_cast<size_t>(rp.m));
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(kMaxJobs, budget / perjob));
+    for (size_t c0 = g0; c0 < g1; c0 += chunk) {
+      const size_t c1 = std::min(g1, c0 + chunk);
+      HOLO_TRY(ensure_ws(h, perjob * (c1 - c0)));
+      std::vector<float2*> F(c1 - c0);
+      int cmax = 0;
+      for (size_t q = c0; q < c1; ++q) {
+        const holo_job& j = jobs[order[q]];
+        F[q - c0] = reinterpret_cast<float2*>(static_cast<char*>(h->ws) + perjob * (q - c0));
+        HOLO_TRY(launch_fill(j.proj, plane, 0x7fffffffu, st));
+        hipEvent_t pe = prof_begin(h, st);
+        HOLO_TRY(forward_spectrum(g, Ex, Ey, j.src, j.src_real, F[q - c0], st));
+        prof_end(h, st, pe, HOLO_PASS_FWD_FFT, 1);
+        cmax = std::max(cmax, j.slices);
+      }
+      for (int b0 = 0; b0 < cmax; b0 += rp.m) {
+        JobTable jt{};
+        int maxnb = 0;
+        for (size_t q = c0; q < c1; ++q) {
+          const holo_job& j = jobs[order[q]];
+          if (j.slices <= b0) continue;
+          JobDesc& d = jt.d[jt.n++];
+          d.F = F[q - c0];
+          d.G = F[q - c0] + plane;
+          d.proj = j.proj;
+          d.zl0 = (j.front.z + static_cast<double>(b0) * j.dz.z) / j.front.wavlen;
+          d.dzl = j.dz.z / j.front.wavlen;
+          d.nbatch = std::min(rp.m, j.slices - b0);
+          maxnb = std::max(maxnb, d.nbatch);
+        }
+        HOLO_TRY(launch_round(h, g, Ex, Ey, rp, j0.front.wavlen, j0.front.dx, jt, maxnb, st));
+      }
+    }
+    g0 = g1;
+  }
+  return HOLO_OK;
+}
+
+int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, double dx, int* out5) {
+  if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
+  if (!(wavlen > 0) || !(dx > 0)) return fail(HOLO_ERR_ARG, "wavlen and dx must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const Tuning saved_global = h->tune;
+  h->tune = Tuning();
+  const size_t plane = static_cast<size_t>(nx) * ny;
+  const size_t pbytes = plane * sizeof(float2);
+  hipEvent_t e0 = ev_get(h), e1 = ev_get(h);
+  if (!e0 || !e1) return fail(HOLO_ERR_HIP, "event creation failed");
+  // time one round of one hologram with m slices under tuning t; returns ms per slice
+  auto time_round = [&](const Tuning& t, bool a, bool b, double* ms_per_slice) -> int {
+    h->size_tune[{nx, ny}] = t;
+    Geom g;
+    int Ex, Ey;
+    int r = make_geom(h, nx, ny, &g, &Ex, &Ey, 512);
+    if (r != HOLO_OK) return r;
+    const ReconPlan rp = recon_plan(h, g, t.batch);
+    HOLO_TRY(ensure_ws(h, pbytes * (1 + static_cast<size_t>(rp.m)) + plane * sizeof(float)));
+    float2* F = static_cast<float2*>(h->ws);
+    float* proj = reinterpret_cast<float*>(F + plane * (1 + rp.m));
+    JobTable jt{};
+    jt.n = 1;
+    jt.d[0] = JobDesc{F, F + plane, proj, -1e4 / wavlen, -200.0 / wavlen, rp.m};
+    HOLO_TRY(launch_round(h, g, Ex, Ey, rp, wavlen, dx, jt, rp.m, st, a, b));  // warm-up
+    HOLO_HIP(hipEventRecord(e0, st));
+    constexpr int kReps = 3;
+    for (int i = 0; i < kReps; ++i) HOLO_TRY(launch_round(h, g, Ex, Ey, rp, wavlen, dx, jt, rp.m, st, a, b));
+    HOLO_HIP(hipEventRecord(e1, st));
+    HOLO_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HOLO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_slice = ms / (kReps * rp.m);
+    return HOLO_OK;
+  };
+  // a synthetic spectrum: any field of the right size (the kernels' cost does not depend on it)
+  HOLO_TRY(ensure_ws(h, pbytes * 2));
+  hipLaunchKernelGGL(k_transfer_natural, dim3((nx + 255) / 256, ny), dim3(256), 0, st, nx, ny, 1.0e3 / wavlen,
+                     wavlen, dx, static_cast<float2*>(h->ws));
+  HOLO_TRY(check_launch());
+  Tuning best;
+  double best_total = 1e30;
+  const size_t mb[2] = {128ull << 20, 256ull << 20};
+  for (size_t bi = 0; bi < 2; ++bi) {
+    Tuning t;
+    t.batch = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, mb[bi] / pbytes)));
+    if (bi == 1 && t.batch == best.batch) continue;
+    double ba = 1e30, bb = 1e30;
+    Tuning ta = t, tb = t;
+    for (int W : {2, 4, 8, 16})
+      for (int sc : {4, 8, 16}) {
+        Tuning c = t;
+        c.cols = W;
+        c.spg_cols = std::min(sc, t.batch);
+        c.spg_rows = 8;
+        double v;
+        if (time_round(c, true, false, &v) != HOLO_OK) continue;
+        if (v < ba) { ba = v; ta = c; }
+      }
+    for (int rows : {1, 2, 4, 8})
+      for (int sr : {4, 8, 16}) {
+        Tuning c = ta;
+        c.rows = rows;
+        c.spg_rows = std::min(sr, t.batch);
+        double v;
+        if (time_round(c, false, true, &v) != HOLO_OK) continue;
+        if (v < bb) { bb = v; tb = c; }
+      }
+    if (ba + bb < best_total) {
+      best_total = ba + bb;
+      best = tb;
+    }
+  }
+  g_err.clear();
+  h->tune = saved_global;
+  if (best_total >= 1e29) {
+    h->size_tune.erase({nx, ny});
+    return fail(HOLO_ERR_SIZE, "no tile configuration fits this size");
+  }
+  h->size_tune[{nx, ny}] = best;
+  if (out5) {
+    out5[0] = best.batch;
+    out5[1] = best.spg_cols;
+    out5[2] = best.spg_rows;
+    out5[3] = best.cols;
+    out5[4] = best.rows;
   }
   return HOLO_OK;
 }

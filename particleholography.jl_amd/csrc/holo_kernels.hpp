@@ -106,6 +106,17 @@ __global__ void k_transfer(int nx, int ny, int mode, double two_pi_z_over_l, dou
   out[idx] = h;
 }
 
+// Natural-order (ifftshifted) H(z) from FP64-wrapped phases; evanescent components 0.
+__global__ void k_transfer_natural(int nx, int ny, double zl, double wavlen, double dx,
+                                   float2* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  const double d = (1.0 - axis_a(i, nx, wavlen, dx)) - axis_a(j, ny, wavlen, dx);
+  out[static_cast<size_t>(j) * nx + i] = d >= 0.0 ? cis_cycles(zl * sqrt(d)) : make_float2(0.f, 0.f);
+}
+
 // psi = ComplexF32(sqrt(holo) + 0im)  (holofunc.jl:80-82)
 __global__ void k_gabor(size_t n, const float* __restrict__ holo, float2* __restrict__ out) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -236,6 +247,23 @@ struct RecArgs {
   int first;             // HMODE 1: this batch starts at slice 0 (R_0 = F .* H0)
 };
 
+// Per-hologram parameters of one round of the generated-H reconstruction. A launch covers
+// up to kMaxJobs holograms of one size (blockIdx.z = entry); the table travels in the kernel
+// arguments. Entries with nbatch = 0 have no slices left in this round.
+constexpr int kMaxJobs = 32;
+struct JobDesc {
+  const float2* F;  // natural-order spectrum
+  float2* G;        // this hologram's batch of column-transformed slices
+  float* proj;      // its projection
+  double zl0;       // z / wavlen of its first slice in this round
+  double dzl;       // dz / wavlen
+  int nbatch;       // its slices in this round
+};
+struct JobTable {
+  int n;
+  JobDesc d[kMaxJobs];
+};
+
 __device__ __forceinline__ size_t centred_index(int i, int j, int nx, int ny) {
   const int ic = (i + (nx >> 1)) % nx;
   const int jc = (j + (ny >> 1)) % ny;
@@ -243,22 +271,28 @@ __device__ __forceinline__ size_t centred_index(int i, int j, int nx, int ny) {
 }
 
 template <int E, int L, int HMODE>
-__global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, Geom g) {
+__global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom g) {
   extern __shared__ float2 lds[];
+  // HMODE 0: hologram blockIdx.z of the table; workgroups past its slice count leave at once
+  const JobDesc& jd = jt.d[HMODE == 0 ? blockIdx.z : 0];
+  if (HMODE == 0 && static_cast<int>(blockIdx.y) * a.spg >= jd.nbatch) return;
+  const float2* __restrict__ Fsrc = HMODE == 0 ? jd.F : a.F;
   const ColCtx c = col_ctx<E, L>(g, lds);
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
   float2 f[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) f[e] = ok ? a.F[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
+  for (int e = 0; e < E; ++e) f[e] = ok ? Fsrc[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
 
   if constexpr (HMODE == 0) {
     // f <- F .* H_{k0} and r = exp(2 pi i dzl sqrt(d)), both from FP64-wrapped phases; then
     // every further slice of the group advances f <- f .* r (one complex multiply).
     const int k0 = blockIdx.y * a.spg;
-    const int k1 = min(k0 + a.spg, a.nbatch);
-    const double zl = a.gp.zl0 + k0 * a.gp.dzl;
+    const int k1 = min(k0 + a.spg, jd.nbatch);
+    const double zl = jd.zl0 + k0 * jd.dzl;
+    const double dzl = jd.dzl;
+    float2* __restrict__ G = jd.G;
     float2 r[E];
     {
       const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
@@ -269,7 +303,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, Geom g) {
         if (d >= 0.0) {
           const double sq = sqrt(d);
           f[e] = cmul(f[e], cis_cycles(zl * sq));
-          r[e] = cis_cycles(a.gp.dzl * sq);
+          r[e] = cis_cycles(dzl * sq);
         } else {  // evanescent component: H = 0
           f[e] = make_float2(0.f, 0.f);
           r[e] = make_float2(0.f, 0.f);
@@ -286,7 +320,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, Geom g) {
       }
       fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
       if (ok) {
-        float2* __restrict__ dst = a.G + static_cast<size_t>(k) * a.gstride;
+        float2* __restrict__ dst = G + static_cast<size_t>(k) * a.gstride;
 #pragma unroll
         for (int e = 0; e < E; ++e) dst[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
       }
@@ -345,11 +379,14 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, size_t n, unsigned int 
 // Projection epilogue (A6): proj = min over slices of |psi_k|^2, psi_k = IFFT/(nx*ny). The
 // minimum follows CUDA.jl's device min for Float32 (fminf: a NaN operand is ignored).
 template <int E, int L>
-__global__ __launch_bounds__(512) void k_recon_rows_proj(const float2* __restrict__ G,
-                                                         size_t gstride, int nbatch, int spg,
-                                                         float* __restrict__ proj, float sc,
-                                                         Geom g) {
+__global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gstride, int spg,
+                                                         float sc, Geom g) {
   extern __shared__ float2 lds[];
+  const JobDesc& jd = jt.d[blockIdx.z];
+  const int nbatch = jd.nbatch;
+  if (static_cast<int>(blockIdx.y) * spg >= nbatch) return;
+  const float2* __restrict__ G = jd.G;
+  float* __restrict__ proj = jd.proj;
   const RowCtx c = row_ctx<E, L>(g, lds);
   const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
@@ -462,11 +499,14 @@ __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
 // Propagation column pass (GS, cu_asm_prop!): forward column FFT, .* H, inverse column FFT.
 // HMODE 0: H generated (FP64-wrapped, natural order); HMODE 1: centred ComplexF32 array.
 // ------------------------------------------------------------------------------------------
+// HMODE 2: natural-order ComplexF32 array (precomputed once per GS call), multiplied as is
+// or conjugated (H(-z) = conj(H(z)) for propagating components).
 struct PropArgs {
   const float2* in;
   float2* out;
   GenParams gp;        // HMODE 0: zl0 = z / wavlen
-  const float2* H;     // HMODE 1
+  const float2* H;     // HMODE 1: centred array; HMODE 2: natural-order array
+  int conj;            // HMODE 2: multiply by conj(H)
 };
 
 template <int E, int L, int HMODE>
@@ -490,9 +530,15 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, Geom g) {
       const float2 h = d >= 0.0 ? cis_cycles(a.gp.zl0 * sqrt(d)) : make_float2(0.f, 0.f);
       v[e] = cmul(v[e], h);
     }
-  } else {
+  } else if constexpr (HMODE == 1) {
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = cmul(v[e], a.H[centred_index(ci, c.t + T * e, g.nx, g.ny)]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 h = a.H[static_cast<size_t>(c.t + T * e) * g.nx + ci];
+      v[e] = a.conj ? cmulconj(v[e], h) : cmul(v[e], h);
+    }
   }
   fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if (ok) {
@@ -522,7 +568,10 @@ __global__ __launch_bounds__(512) void k_gs_rows(const float2* __restrict__ U,
     const float A = c.ok ? sqrtf(I[rowoff + c.t + T * e]) : 0.f;
     const float a = v[e].x * sc, b = v[e].y * sc;
     const float r2 = a * a + b * b;
-    if (r2 > 0.f) {
+    if (r2 >= 1e-30f) {  // v_rsq_f32 (1 ulp) away from the denormal range
+      const float f = A * __builtin_amdgcn_rsqf(r2);
+      v[e] = make_float2(a * f, b * f);
+    } else if (r2 > 0.f) {
       const float f = A / sqrtf(r2);
       v[e] = make_float2(a * f, b * f);
     } else {

@@ -30,6 +30,12 @@ class holo_tf(ctypes.Structure):
                 ("dx", ctypes.c_double), ("generated", ctypes.c_int)]
 
 
+class holo_job(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("src", ctypes.c_void_p),
+                ("src_real", ctypes.c_int), ("front", holo_tf), ("dz", holo_tf),
+                ("slices", ctypes.c_int), ("proj", ctypes.c_void_p)]
+
+
 _vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
 _tfp = ctypes.POINTER(holo_tf)
 
@@ -52,6 +58,10 @@ SIGNATURES = {
     "holo_phase_retrieval": [_vp, _vp, _i, _i, _vp, _vp, _tfp, _tfp, _i, _vp, _i],
     "holo_asm_prop": [_vp, _vp, _i, _i, _vp, _vp, _tfp],
     "holo_fft2": [_vp, _vp, _i, _i, _vp, _vp, _i],
+    "holo_reconst_batch": [_vp, _vp, ctypes.POINTER(holo_job), _i],
+    "holo_set_tuning_for_size": [_vp, _i, _i, _i, _i, _i, _i, _i],
+    "holo_get_tuning_for_size": [_vp, _i, _i, _i, ctypes.POINTER(ctypes.c_int)],
+    "holo_autotune": [_vp, _vp, _i, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
     "holo_set_profiling": [_vp, _i],
     "holo_profile_read": [_vp, _i, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong),
                           ctypes.POINTER(ctypes.c_longlong), _i],

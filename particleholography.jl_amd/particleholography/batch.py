@@ -1,0 +1,115 @@
+"""Batched reconstruction of many holograms of uneven sizes, and the tile autotuner.
+
+``reconst_batch`` runs many ``cu_get_reconst_xyprojection`` calls (holofunc.jl:248-266)
+together through ``holo_reconst_batch``: holograms are grouped by size, and every round of
+slices of a group is one column-pass launch and one row-pass launch over all of its holograms.
+Each projection equals the one the single-hologram call returns.
+
+``autotune`` times candidate tile configurations of both reconstruction passes on the device
+for one size and keeps the fastest in the library handle; results are cached per device name
+in a JSON file (``$PARTICLEHOLOGRAPHY_TUNING`` or ``~/.cache/particleholography/tuning.json``)
+and re-applied by ``load_tuning``.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import List, Optional
+
+import torch
+
+from . import _lib
+from ._lib import check, holo_job, holo_tf
+
+
+@dataclass
+class ReconJob:
+    """One hologram to reconstruct: Float32 intensity (ny, nx) or ComplexF32 wavefront."""
+    holo: torch.Tensor
+    z_front: float
+    z_dz: float
+    slices: int
+    wavlen: float = 0.6328
+    dx: float = 10.0
+    proj: Optional[torch.Tensor] = None
+
+
+def reconst_batch(jobs: List[ReconJob], device: int = 0) -> List[torch.Tensor]:
+    """xy min-projections of every job (Float32 (ny, nx) tensors), computed together."""
+    arr = (holo_job * max(1, len(jobs)))()
+    keep = []
+    for i, j in enumerate(jobs):
+        src = j.holo.contiguous()
+        if src.dtype not in (torch.float32, torch.complex64):
+            raise AssertionError(f"job {i}: hologram must be float32 or complex64, got {src.dtype}")
+        if not src.is_cuda:
+            raise AssertionError(f"job {i}: arrays must be device (GPU) tensors")
+        ny, nx = src.shape
+        if j.proj is None:
+            j.proj = torch.empty((ny, nx), dtype=torch.float32, device=src.device)
+        assert tuple(j.proj.shape) == (ny, nx) and j.proj.dtype == torch.float32, f"job {i}: bad projection"
+        keep.append(src)
+        arr[i].nx, arr[i].ny = nx, ny
+        arr[i].src = src.data_ptr()
+        arr[i].src_real = 1 if src.dtype == torch.float32 else 0
+        arr[i].front = holo_tf(None, float(j.z_front), float(j.wavlen), float(j.dx), 1)
+        arr[i].dz = holo_tf(None, float(j.z_dz), float(j.wavlen), float(j.dx), 1)
+        arr[i].slices = int(j.slices)
+        arr[i].proj = j.proj.data_ptr()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    check(_lib.load().holo_reconst_batch(_lib.handle(device).ptr, stream, arr, len(jobs)))
+    return [j.proj for j in jobs]
+
+
+def _cache_path() -> Path:
+    p = os.environ.get("PARTICLEHOLOGRAPHY_TUNING")
+    return Path(p) if p else Path.home() / ".cache" / "particleholography" / "tuning.json"
+
+
+def _device_key(device: int) -> str:
+    return torch.cuda.get_device_name(device)
+
+
+FIELDS = ("batch", "spg_cols", "spg_rows", "cols_per_wg", "rows_per_wg")
+
+
+def autotune(nx: int, ny: int, wavlen: float = 0.6328, dx: float = 10.0, device: int = 0,
+             save: bool = True) -> dict:
+    """Times candidate tiles for an (nx, ny) reconstruction and keeps the fastest."""
+    out = (ctypes.c_int * 5)()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    check(_lib.load().holo_autotune(_lib.handle(device).ptr, stream, nx, ny, float(wavlen), float(dx), out))
+    choice = dict(zip(FIELDS, list(out)))
+    if save:
+        path = _cache_path()
+        data = json.loads(path.read_text()) if path.exists() else {}
+        data.setdefault(_device_key(device), {})[f"{nx}x{ny}"] = choice
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.write_text(json.dumps(data, indent=1, sort_keys=True))
+    return choice
+
+
+def set_tuning_for_size(nx: int, ny: int, choice: dict, device: int = 0) -> None:
+    vals = [int(choice.get(f, 0)) for f in FIELDS]
+    check(_lib.load().holo_set_tuning_for_size(_lib.handle(device).ptr, nx, ny, *vals))
+
+
+def get_tuning_for_size(nx: int, ny: int, slices: int = 0, device: int = 0) -> dict:
+    out = (ctypes.c_int * 5)()
+    check(_lib.load().holo_get_tuning_for_size(_lib.handle(device).ptr, nx, ny, int(slices), out))
+    return dict(zip(FIELDS, list(out)))
+
+
+def load_tuning(device: int = 0) -> int:
+    """Applies cached autotuner choices for this device; returns how many sizes were set."""
+    path = _cache_path()
+    if not path.exists():
+        return 0
+    entries = json.loads(path.read_text()).get(_device_key(device), {})
+    for size, choice in entries.items():
+        nx, ny = (int(v) for v in size.split("x"))
+        set_tuning_for_size(nx, ny, choice, device)
+    return len(entries)

@@ -52,6 +52,9 @@ def test_holo_tf_struct_layout():
 
     assert ctypes.sizeof(_lib.holo_tf) == 40
     assert _lib.holo_tf.z.offset == 8 and _lib.holo_tf.generated.offset == 32
+    assert ctypes.sizeof(_lib.holo_job) == 120
+    assert _lib.holo_job.front.offset == 24 and _lib.holo_job.slices.offset == 104
+    assert _lib.holo_job.proj.offset == 112
 
 
 def test_mirror_validates_sizes_before_device_work():

@@ -1,0 +1,98 @@
+"""Batched reconstruction of holograms of uneven sizes (one column-pass and one row-pass launch
+per round over all holograms of a size) and the tile autotuner, through the C ABI."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import asm
+
+pytestmark = pytest.mark.gpu
+
+LAM, DX = 0.6328, 10.0
+
+
+def _holo(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    return (1.0 + 0.2 * rng.standard_normal((ny, nx))).clip(0.0).astype(np.float32)
+
+
+def _single(ph, h, z0, dz, slices, nx, ny):
+    import ctypes
+    from particleholography import _lib
+
+    proj = torch.empty((ny, nx), dtype=torch.float32, device=h.device)
+    f = _lib.holo_tf(None, z0, LAM, DX, 1)
+    d = _lib.holo_tf(None, dz, LAM, DX, 1)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(_lib.load().holo_reconst_xyprojection_holo(_lib.handle(0).ptr, st, nx, ny, ctypes.c_void_p(h.data_ptr()),
+                                                          ctypes.byref(f), ctypes.byref(d), slices,
+                                                          ctypes.c_void_p(proj.data_ptr())))
+    return proj
+
+
+def test_batch_uneven_sizes_equal_single_calls(gpu):
+    import particleholography as ph
+    from particleholography.batch import ReconJob, reconst_batch
+
+    cases = [(512, 512, 17, -1e4, -700.0), (768, 768, 40, -2e4, -300.0), (1024, 1024, 33, -1e4, -200.0),
+             (1280, 1024, 8, -5e4, -1000.0), (256, 256, 25, -8e4, -100.0), (1024, 1024, 60, -3e4, -150.0),
+             (512, 512, 3, -1e4, -50.0)]
+    jobs, holos = [], []
+    for i, (nx, ny, s, z0, dz) in enumerate(cases):
+        h = torch.from_numpy(_holo(nx, ny, 100 + i)).to(gpu)
+        holos.append(h)
+        jobs.append(ReconJob(h, z0, dz, s, LAM, DX))
+    outs = reconst_batch(jobs)
+    for (nx, ny, s, z0, dz), h, out in zip(cases, holos, outs):
+        want = _single(ph, h, z0, dz, s, nx, ny)
+        np.testing.assert_array_equal(out.cpu().numpy(), want.cpu().numpy())
+
+
+def test_batch_vs_fp64_oracle_and_chunking(gpu):
+    from particleholography.batch import ReconJob, reconst_batch
+
+    rng = np.random.default_rng(5)
+    jobs, hs = [], []
+    for i in range(37):  # more than one 32-hologram launch
+        n = 64 if i % 2 == 0 else 96
+        s = int(rng.integers(1, 9))
+        h = _holo(n, n, 300 + i)
+        hs.append((h, s))
+        jobs.append(ReconJob(torch.from_numpy(h).to(gpu), -1e4 - 100.0 * i, -250.0, s, LAM, DX))
+    outs = reconst_batch(jobs)
+    for (h, s), j, out in zip(hs, jobs, outs):
+        want = asm.reconst_xyprojection(asm.gabor_wavefront(h), s, "exact", z_front=j.z_front, z_dz=j.z_dz,
+                                        wavlen=LAM, dx=DX)
+        assert asm.rel_l2(out.cpu().numpy(), want) < 1e-4
+
+
+def test_batch_rejects_array_transfers_and_bad_jobs(gpu):
+    import ctypes
+    from particleholography import _lib
+
+    arr = (_lib.holo_job * 1)()
+    arr[0].nx = arr[0].ny = 64
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    with pytest.raises(AssertionError, match="NULL"):
+        _lib.check(_lib.load().holo_reconst_batch(_lib.handle(0).ptr, st, arr, 1))
+    assert _lib.load().holo_reconst_batch(_lib.handle(0).ptr, st, arr, 0) == 0
+
+
+def test_autotune_keeps_results_correct(gpu):
+    import particleholography as ph
+    from particleholography import batch
+
+    n = 256
+    choice = batch.autotune(n, n, LAM, DX, save=False)
+    assert choice["batch"] >= 1 and choice["cols_per_wg"] in (1, 2, 4, 8, 16)
+    eff = batch.get_tuning_for_size(n, n, 1000)
+    assert eff["batch"] == choice["batch"] and eff["spg_cols"] == choice["spg_cols"]
+    try:
+        h = _holo(n, n, 9)
+        ht = torch.from_numpy(h).to(gpu)
+        got = _single(ph, ht, -2e4, -300.0, 40, n, n).cpu().numpy()
+        want = asm.reconst_xyprojection(asm.gabor_wavefront(h), 40, "exact", z_front=-2e4, z_dz=-300.0,
+                                        wavlen=LAM, dx=DX)
+        assert asm.rel_l2(got, want) < 1e-4
+    finally:
+        batch.set_tuning_for_size(n, n, {})
