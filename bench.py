@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the plain-path and GS figures")
+    ap.add_argument("--batch-holograms", type=int, default=64, help="holograms in the batched workload")
+    ap.add_argument("--serve-requests", type=int, default=1000,
+                    help="requests in the served Poisson stream (the full stream has 5000)")
     ap.add_argument("--profile-gs", action="store_true",
                     help="for rocprofv3: Gerchberg-Saxton at 1024^2 (one warm-up call, then --steps "
                          "calls of 20 iterations: eager launches, then the hipGraph form); no extras")
@@ -159,14 +162,9 @@ def main():
         plain = 2 * SLICES / pw
         extras["plain_path_slices_per_s"] = round(plain, 1)
         extras["speedup_vs_plain"] = round(slices_per_s / plain, 2)
-        # Gerchberg-Saxton at 1024^2: 100 iterations, hipGraph-captured iteration
-        i1 = holo
-        i2 = torch.from_numpy(synth.gs_pair(103, 500, N, 2000.0)[1]).to(dev)
-        hf, hb = ph.cu_transfer(2000.0, N, LAM, d_sqr), ph.cu_transfer(-2000.0, N, LAM, d_sqr)
-        gs = lambda: ph.cu_phase_retrieval_holo(i1, i2, hf, hb, 100, N)  # noqa: E731
-        gw, _ = timed_steps(gs, 3, 1, torch)
-        extras["gs_iterations_per_s"] = round(300 / gw, 1)
-        extras["gs_algorithmic_GBs"] = round(72.0 * N * N * 300 / gw / 1e9, 1)
+        extras["gs_1024"] = bench_gs(ph, synth, torch, dev, d_sqr)
+        extras["batched_64"] = bench_batched(torch, dev, args.batch_holograms)
+        extras["serving"] = bench_serving(torch, args.serve_requests)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -188,6 +186,92 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "extras": extras,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_gs(ph, synth, torch, dev, d_sqr):
+    """Gerchberg-Saxton at 1024^2: a seeded 500-disk scene recorded at the sensor and 2000 um
+    further, 100 iterations (hipGraph-captured iteration), then a 512-slice reconstruction."""
+    i1_np, i2_np = synth.gs_pair(103, 500, N, 2000.0)
+    i1, i2 = torch.from_numpy(i1_np).to(dev), torch.from_numpy(i2_np).to(dev)
+    hf, hb = ph.cu_transfer(2000.0, N, LAM, d_sqr), ph.cu_transfer(-2000.0, N, LAM, d_sqr)
+    h0, hdz = ph.cu_transfer(-Z0, N, LAM, d_sqr), ph.cu_transfer(-DZ, N, LAM, d_sqr)
+
+    def run():
+        wf = ph.cu_phase_retrieval_holo(i1, i2, hf, hb, 100, N)
+        return ph.cu_get_reconst_xyprojection(wf, h0, hdz, 512)
+
+    gw, _ = timed_steps(lambda: ph.cu_phase_retrieval_holo(i1, i2, hf, hb, 100, N), 3, 1, torch)
+    tw, _ = timed_steps(run, 3, 1, torch)
+    return {"iterations_per_s": round(300 / gw, 1), "gs_algorithmic_GBs": round(72.0 * N * N * 300 / gw / 1e9, 1),
+            "pipeline_ms": round(tw / 3 * 1e3, 3), "config": "1024^2, 100 iterations at 2000 um, then 512 slices"}
+
+
+def bench_batched(torch, dev, nholo):
+    """64 seeded holograms, sides from {512, 768, 1024, 1280x1024, 2048}, 100-2000 disks,
+    128-1024 slices each over z 1e4-2e5 um; one batched call (tiles autotuned per size)."""
+    import numpy as np
+
+    from particleholography import batch, synth
+    from particleholography.serve import SIZES
+
+    rng = np.random.Generator(np.random.PCG64(104))
+    jobs = []
+    for i in range(nholo):
+        nx, ny = SIZES[int(rng.integers(len(SIZES)))]
+        nd = int(rng.integers(100, 2001))
+        s = int(rng.integers(128, 1025))
+        h = synth.hologram_device(104 * 1000 + i, nd, nx, ny, zmin=1e4, zmax=2e5)
+        jobs.append(batch.ReconJob(h, -1e4, -(2e5 - 1e4) / s, s, LAM, DX))
+    tuned = {f"{nx}x{ny}": batch.autotune(nx, ny, LAM, DX, save=False) for nx, ny in SIZES}
+    batch.reconst_batch(jobs)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.reconst_batch(jobs)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    slices = sum(j.slices for j in jobs)
+    px_slices = sum(j.slices * j.holo.numel() for j in jobs)
+    out = {"holograms": nholo, "slices": slices, "slices_per_s": round(slices / dt, 1),
+           "slices_1024eq_per_s": round(px_slices / (N * N) / dt, 1),
+           "algorithmic_GBs": round(32.0 * px_slices / dt / 1e9, 1), "wall_ms": round(dt * 1e3, 2),
+           "tuning": tuned, "per_size": {}}
+    for nx, ny in SIZES:
+        sub = [j for j in jobs if tuple(j.holo.shape) == (ny, nx)]
+        if not sub:
+            continue
+        batch.reconst_batch(sub)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch.reconst_batch(sub)
+        torch.cuda.synchronize()
+        d = time.perf_counter() - t0
+        ss = sum(j.slices for j in sub)
+        gbs = 32.0 * nx * ny * ss / d / 1e9
+        out["per_size"][f"{nx}x{ny}"] = {"holograms": len(sub), "slices": ss, "slices_per_s": round(ss / d, 1),
+                                         "algorithmic_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
+    for nx, ny in SIZES:
+        batch.set_tuning_for_size(nx, ny, {})
+    return out
+
+
+def bench_serving(torch, nreq):
+    """Seeded Poisson stream of mixed 512^2-2048^2 requests (128-1024 slices, 20% two-plane GS
+    with 50 iterations): capacity with every request queued at once, then the stream at 75% of
+    that capacity (p50/p99 latency), beside the plain one-at-a-time path."""
+    from particleholography import serve
+
+    reqs = serve.make_stream(nreq, rate=0.0, seed=105)
+    srv = serve.Server(n_slots=4, max_batch=16, hbm_budget=64 << 30, workspace_per_slot=4 << 30)
+    srv.run(reqs[: min(32, nreq)])  # warm-up: plans, workspaces
+    cap = srv.run(reqs)
+    rate = 0.75 * cap["requests"] / cap["makespan_s"]
+    reqs2 = serve.make_stream(nreq, rate=rate, seed=105)
+    live = srv.run(reqs2)
+    plain = serve.plain_one_at_a_time(reqs)
+    r = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
+    return {"capacity": r(cap), "poisson_at_75pct": r(live), "rate_req_per_s": round(rate, 2),
+            "plain_one_at_a_time": r(plain),
+            "speedup_vs_plain": round(cap["slices_per_s"] / plain["slices_per_s"], 2)}
 
 
 if __name__ == "__main__":

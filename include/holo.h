@@ -163,6 +163,23 @@ typedef struct holo_job {
  * 32 at a time; workspace of (1 + batch) planes per hologram, chunked to HOLO_BATCH_WS_BYTES).
  * Each result equals the single-hologram call's. */
 int holo_reconst_batch(holo_handle h, void* stream, const holo_job* jobs, int njobs);
+/* Batched Gerchberg-Saxton: many cu_phase_retrieval_holo calls (holofunc.jl:100-129) together.
+ * Pairs are grouped by size; each pass of an iteration is one launch over up to 32 pairs.
+ * Pairs may have different transfers and iteration counts. Each result equals the single
+ * call's. */
+typedef struct holo_gs_job {
+  int nx, ny;
+  const float* holo1;
+  const float* holo2;
+  holo_tf fwd;       /* holo1 -> holo2 */
+  holo_tf bwd;       /* holo2 -> holo1 */
+  int iters;
+  void* out;         /* ComplexF32 nx*ny */
+} holo_gs_job;
+int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* jobs, int njobs);
+/* Caps the workspace the batched calls plan for (holograms are processed in chunks that fit;
+ * a single hologram larger than the cap still runs). 0 = HOLO_BATCH_WS_BYTES or 8 GiB. */
+int holo_set_workspace_limit(holo_handle h, size_t bytes);
 /* Per-size tile choices (0 = default): slices per batch, slices per column / row workgroup,
  * columns per workgroup, rows per workgroup. The global holo_set_tuning overrides them. */
 int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_cols, int spg_rows,

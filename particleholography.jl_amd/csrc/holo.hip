@@ -113,6 +113,7 @@ struct holo_context {
   size_t ws_bytes = 0;
   Tuning tune;                                    // global override (tests, sweeps)
   std::map<std::pair<int, int>, Tuning> size_tune;  // per-size choices (autotuner)
+  size_t batch_ws_limit = 0;                      // batched calls' workspace cap (0 = default)
   hipStream_t cap = nullptr;
   double2* partials = nullptr;  // 256 partial sums for cu_2d_pad's mean
   struct GsKey {
@@ -602,72 +603,132 @@ int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan
   return HOLO_OK;
 }
 
-// One propagation column pass: in -> FFT_col -> .* H -> IFFT_col -> out. With Hnat != NULL
-// the natural-order array is used (conjugated when conj != 0) instead of `t`.
+// One propagation column pass over a single plane: in -> FFT_col -> .* H(t) -> IFFT_col -> out
+// (H generated per element, or read from the centred array; cu_asm_prop!).
 int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, float2* out,
-                     hipStream_t st, const float2* Hnat = nullptr, int conj = 0) {
+                     hipStream_t st) {
   PropArgs pa{};
   pa.in = in;
   pa.out = out;
-  const int mode = Hnat ? 2 : (t->generated ? 0 : 1);
-  if (mode == 2) {
-    pa.H = Hnat;
-    pa.conj = conj;
-  } else if (mode == 0) {
+  const int mode = t->generated ? 0 : 1;
+  if (mode == 0) {
     pa.gp.wavlen = t->wavlen;
     pa.gp.dx = t->dx;
     pa.gp.zl0 = t->z / t->wavlen;
   } else {
     pa.H = static_cast<const float2*>(t->data);
   }
+  GsTable gt{};
   const size_t lds = col_lds(g);
   bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
     auto launch = [&](auto kern) {
       allow_lds(kern, lds);
-      hipLaunchKernelGGL(kern, col_grid(g), col_block(g), lds, st, pa, g);
+      hipLaunchKernelGGL(kern, col_grid(g), col_block(g), lds, st, pa, gt, g);
     };
     if (mode == 0) launch(k_prop_cols<E, L, 0>);
-    else if (mode == 1) launch(k_prop_cols<E, L, 1>);
-    else launch(k_prop_cols<E, L, 2>);
+    else launch(k_prop_cols<E, L, 1>);
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
   return check_launch();
 }
 
-int launch_gs_rows(const Geom& g, int Ex, int mode, const float2* U, const float* I, float2* out,
-                   float sc, hipStream_t st) {
+// GS column pass for every pair of the table (natural-order transfer arrays).
+int launch_prop_cols_table(const Geom& g, int Ey, const GsTable& gt, hipStream_t st) {
+  PropArgs pa{};
+  const size_t lds = col_lds(g);
+  bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
+    constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
+    allow_lds(k_prop_cols<E, L, 2>, lds);
+    dim3 grid = col_grid(g);
+    grid.z = gt.n;
+    hipLaunchKernelGGL((k_prop_cols<E, L, 2>), grid, col_block(g), lds, st, pa, gt, g);
+  });
+  if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
+  return check_launch();
+}
+
+// GS row pass for every pair of the table.
+int launch_gs_rows_table(const Geom& g, int Ex, const GsTable& gt, float sc, hipStream_t st) {
   const size_t lds = row_lds(g);
   bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
-    if (mode == 0) {
-      allow_lds(k_gs_rows<E, L, 0>, lds);
-      hipLaunchKernelGGL((k_gs_rows<E, L, 0>), row_grid(g), row_block(g), lds, st, U, I, out, sc, g);
-    } else {
-      allow_lds(k_gs_rows<E, L, 1>, lds);
-      hipLaunchKernelGGL((k_gs_rows<E, L, 1>), row_grid(g), row_block(g), lds, st, U, I, out, sc, g);
-    }
+    allow_lds(k_gs_rows<E, L>, lds);
+    dim3 grid = row_grid(g);
+    grid.z = gt.n;
+    hipLaunchKernelGGL((k_gs_rows<E, L>), grid, row_block(g), lds, st, gt, sc, g);
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
   return check_launch();
 }
 
-// One GS iteration. hf / hb: natural-order transfer arrays (hb may equal hf with conj_b set),
-// or NULL to use the transfers as given.
-int gs_iteration(holo_context* h, const Geom& g, int Ex, int Ey, float2* B, const float* I1,
-                 const float* I2, const holo_tf* fwd, const holo_tf* bwd, const float2* hf,
-                 const float2* hb, int conj_b, float sc, bool last, float2* out, hipStream_t st) {
+// State of one pair during GS: its field B, natural-order transfers and iteration count.
+struct GsPair {
+  float2* B;
+  const float2* hf;
+  const float2* hb;
+  int conj_b;
+  const float* I1;
+  const float* I2;
+  float2* out;
+  int iters;
+};
+
+// Iteration `it` of every pair in [p0, p1) that still has iterations left: column pass
+// (+z), row pass with I2, column pass (-z), row pass with I1 (final for a pair's last one).
+int gs_round(holo_context* h, const Geom& g, int Ex, int Ey, const GsPair* pairs, int np, int it,
+             float sc, hipStream_t st) {
+  GsTable f{}, r2{}, b{}, r1{};
+  for (int q = 0; q < np; ++q) {
+    const GsPair& p = pairs[q];
+    if (p.iters <= it) continue;
+    const int k = f.n;
+    f.d[k] = GsDesc{p.B, p.hf, nullptr, nullptr, 0, 0};
+    r2.d[k] = GsDesc{p.B, nullptr, p.I2, nullptr, 0, 0};
+    b.d[k] = GsDesc{p.B, p.hb, nullptr, nullptr, p.conj_b, 0};
+    r1.d[k] = GsDesc{p.B, nullptr, p.I1, p.out, 0, it + 1 == p.iters ? 1 : 0};
+    f.n = r2.n = b.n = r1.n = k + 1;
+  }
+  if (f.n == 0) return HOLO_OK;
+  const GsTable* cols[2] = {&f, &b};
+  const GsTable* rows[2] = {&r2, &r1};
   for (int half = 0; half < 2; ++half) {
     hipEvent_t pc = prof_begin(h, st);
-    HOLO_TRY(launch_prop_cols(g, Ey, half == 0 ? fwd : bwd, B, B, st, half == 0 ? hf : hb,
-                              half == 0 ? 0 : conj_b));
-    prof_end(h, st, pc, HOLO_PASS_PROP_COLS, 1);
+    HOLO_TRY(launch_prop_cols_table(g, Ey, *cols[half], st));
+    prof_end(h, st, pc, HOLO_PASS_PROP_COLS, cols[half]->n);
     hipEvent_t pr = prof_begin(h, st);
-    if (half == 0) HOLO_TRY(launch_gs_rows(g, Ex, 0, B, I2, B, sc, st));
-    else HOLO_TRY(launch_gs_rows(g, Ex, last ? 1 : 0, B, I1, last ? out : B, sc, st));
-    prof_end(h, st, pr, HOLO_PASS_GS_ROWS, 1);
+    HOLO_TRY(launch_gs_rows_table(g, Ex, *rows[half], sc, st));
+    prof_end(h, st, pr, HOLO_PASS_GS_ROWS, rows[half]->n);
+  }
+  return HOLO_OK;
+}
+
+// Natural-order transfer arrays of a pair into hf (and hb unless mirrored): generated
+// transfers are evaluated once (H(-z) = conj H(z) for a mirrored pair), arrays are ifftshifted.
+int gs_transfers(const Geom& g, const holo_tf* fwd, const holo_tf* bwd, float2* hf, float2* hbuf,
+                 const float2** hb, int* conj_b, hipStream_t st) {
+  const dim3 grid((g.nx + 255) / 256, g.ny), block(256);
+  const bool mirrored = fwd->generated && bwd->generated && bwd->z == -fwd->z &&
+                        bwd->wavlen == fwd->wavlen && bwd->dx == fwd->dx;
+  auto make = [&](const holo_tf* t, float2* dst) {
+    if (t->generated)
+      hipLaunchKernelGGL(k_transfer_natural, grid, block, 0, st, g.nx, g.ny, t->z / t->wavlen, t->wavlen, t->dx, dst);
+    else
+      hipLaunchKernelGGL(k_to_natural, grid, block, 0, st, g.nx, g.ny, static_cast<const float2*>(t->data), dst);
+  };
+  make(fwd, hf);
+  HOLO_TRY(check_launch());
+  if (mirrored) {
+    *hb = hf;
+    *conj_b = 1;
+  } else {
+    make(bwd, hbuf);
+    HOLO_TRY(check_launch());
+    *hb = hbuf;
+    *conj_b = 0;
   }
   return HOLO_OK;
 }
@@ -889,34 +950,21 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
   int Ex, Ey;
   HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
   const size_t plane = static_cast<size_t>(nx) * ny;
-  // workspace: [field B | H(fwd) | H(bwd)] -- generated transfers are evaluated once per call
-  // in natural order; H(-z) = conj(H(z)) lets a mirrored pair share one array.
-  const bool gen = fwd->generated && bwd->generated;
-  const bool mirrored = gen && bwd->z == -fwd->z && bwd->wavlen == fwd->wavlen && bwd->dx == fwd->dx;
-  const int narr = gen ? (mirrored ? 1 : 2) : 0;
-  HOLO_TRY(ensure_ws(h, plane * sizeof(float2) * (1 + narr)));
-  float2* B = static_cast<float2*>(h->ws);
-  float2* hf = nullptr;
-  float2* hb = nullptr;
-  int conj_b = 0;
-  if (gen) {
-    hf = B + plane;
-    hb = mirrored ? hf : hf + plane;
-    conj_b = mirrored ? 1 : 0;
-    const dim3 grid((nx + 255) / 256, ny), block(256);
-    hipLaunchKernelGGL(k_transfer_natural, grid, block, 0, st, nx, ny, fwd->z / fwd->wavlen, fwd->wavlen, fwd->dx, hf);
-    HOLO_TRY(check_launch());
-    if (!mirrored) {
-      hipLaunchKernelGGL(k_transfer_natural, grid, block, 0, st, nx, ny, bwd->z / bwd->wavlen, bwd->wavlen, bwd->dx, hb);
-      HOLO_TRY(check_launch());
-    }
-  }
+  HOLO_TRY(ensure_ws(h, plane * sizeof(float2) * 3));  // [field | H fwd | H bwd]
+  GsPair p{};
+  p.B = static_cast<float2*>(h->ws);
+  p.I1 = holo1;
+  p.I2 = holo2;
+  p.out = static_cast<float2*>(out);
+  p.iters = iters;
+  float2* hf = p.B + plane;
+  p.hf = hf;
+  HOLO_TRY(gs_transfers(g, fwd, bwd, hf, hf + plane, &p.hb, &p.conj_b, st));
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
-  float2* o = static_cast<float2*>(out);
-  HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, holo1, 0, B, 0, 1.0f, 1, st));  // psi1 = sqrt(I1), row FFT
-  const int body = iters - 1;
+  HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, holo1, 0, p.B, 0, 1.0f, 1, st));  // psi1 = sqrt(I1), row FFT
+  const int body = iters - 1;  // iterations 0 .. iters-2 are identical; the last one stores psi
   if (use_graph && body > 0) {
-    holo_context::GsKey key{nx, ny, B, holo1, holo2, fwd->data, bwd->data, fwd->z, bwd->z,
+    holo_context::GsKey key{nx, ny, p.B, holo1, holo2, fwd->data, bwd->data, fwd->z, bwd->z,
                             fwd->wavlen, fwd->dx, fwd->generated, bwd->generated, g.W, g.rows};
     hipGraphExec_t exec = nullptr;
     auto it = h->gs_graphs.find(key);
@@ -925,7 +973,7 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
     } else {
       hipGraph_t graph = nullptr;
       HOLO_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
-      const int r = gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, hf, hb, conj_b, sc, false, o, h->cap);
+      const int r = gs_round(h, g, Ex, Ey, &p, 1, 0, sc, h->cap);
       const hipError_t e = hipStreamEndCapture(h->cap, &graph);
       if (r != HOLO_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -939,10 +987,66 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
     }
     for (int it2 = 0; it2 < body; ++it2) HOLO_HIP(hipGraphLaunch(exec, st));
   } else {
-    for (int it2 = 0; it2 < body; ++it2)
-      HOLO_TRY(gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, hf, hb, conj_b, sc, false, o, st));
+    for (int it2 = 0; it2 < body; ++it2) HOLO_TRY(gs_round(h, g, Ex, Ey, &p, 1, it2, sc, st));
   }
-  return gs_iteration(h, g, Ex, Ey, B, holo1, holo2, fwd, bwd, hf, hb, conj_b, sc, true, o, st);
+  return gs_round(h, g, Ex, Ey, &p, 1, iters - 1, sc, st);
+}
+
+int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* jobs, int njobs) {
+  if (!h || (njobs > 0 && !jobs)) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (njobs < 0) return fail(HOLO_ERR_ARG, "njobs must be >= 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (int i = 0; i < njobs; ++i) {
+    const holo_gs_job& j = jobs[i];
+    if (!j.holo1 || !j.holo2 || !j.out) return fail(HOLO_ERR_ARG, "job " + std::to_string(i) + ": NULL argument");
+    if (j.iters < 0) return fail(HOLO_ERR_ARG, "job " + std::to_string(i) + ": iterations must be >= 0");
+    HOLO_TRY(validate_transfer(&j.fwd, "transfer"));
+    HOLO_TRY(validate_transfer(&j.bwd, "invtransfer"));
+  }
+  std::vector<int> order;
+  for (int i = 0; i < njobs; ++i) {
+    if (jobs[i].iters == 0) HOLO_TRY(holo_gabor_wavefront(h, stream, jobs[i].nx, jobs[i].ny, jobs[i].holo1, jobs[i].out));
+    else order.push_back(i);
+  }
+  auto key = [&](int i) { return std::make_pair(jobs[i].nx, jobs[i].ny); };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  const size_t budget = h->batch_ws_limit ? h->batch_ws_limit : env_size("HOLO_BATCH_WS_BYTES", 8ull << 30);
+  for (size_t g0 = 0; g0 < order.size();) {
+    size_t g1 = g0 + 1;
+    while (g1 < order.size() && key(order[g1]) == key(order[g0])) ++g1;
+    const int nx = jobs[order[g0]].nx, ny = jobs[order[g0]].ny;
+    Geom g;
+    int Ex, Ey;
+    HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+    const size_t plane = static_cast<size_t>(nx) * ny;
+    const size_t perpair = 3 * plane * sizeof(float2);
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(kMaxJobs, budget / perpair));
+    const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
+    for (size_t c0 = g0; c0 < g1; c0 += chunk) {
+      const size_t c1 = std::min(g1, c0 + chunk);
+      HOLO_TRY(ensure_ws(h, perpair * (c1 - c0)));
+      std::vector<GsPair> pairs(c1 - c0);
+      int maxit = 0;
+      for (size_t q = c0; q < c1; ++q) {
+        const holo_gs_job& j = jobs[order[q]];
+        GsPair& p = pairs[q - c0];
+        p.B = reinterpret_cast<float2*>(static_cast<char*>(h->ws) + perpair * (q - c0));
+        float2* hf = p.B + plane;
+        p.hf = hf;
+        HOLO_TRY(gs_transfers(g, &j.fwd, &j.bwd, hf, hf + plane, &p.hb, &p.conj_b, st));
+        p.I1 = j.holo1;
+        p.I2 = j.holo2;
+        p.out = static_cast<float2*>(j.out);
+        p.iters = j.iters;
+        HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, j.holo1, 0, p.B, 0, 1.0f, 1, st));
+        maxit = std::max(maxit, j.iters);
+      }
+      for (int it = 0; it < maxit; ++it)
+        HOLO_TRY(gs_round(h, g, Ex, Ey, pairs.data(), static_cast<int>(pairs.size()), it, sc, st));
+    }
+    g0 = g1;
+  }
+  return HOLO_OK;
 }
 
 int holo_set_profiling(holo_handle h, int enable) {
@@ -1017,7 +1121,7 @@ int holo_reconst_batch(holo_handle h, void* stream, const holo_job* jobs, int nj
     return std::make_tuple(jobs[i].nx, jobs[i].ny, jobs[i].front.wavlen, jobs[i].front.dx);
   };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
-  const size_t budget = env_size("HOLO_BATCH_WS_BYTES", 8ull << 30);
+  const size_t budget = h->batch_ws_limit ? h->batch_ws_limit : env_size("HOLO_BATCH_WS_BYTES", 8ull << 30);
   for (size_t g0 = 0; g0 < order.size();) {
     size_t g1 = g0 + 1;
     while (g1 < order.size() && key(order[g1]) == key(order[g0])) ++g1;
@@ -1155,6 +1259,12 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     out5[3] = best.cols;
     out5[4] = best.rows;
   }
+  return HOLO_OK;
+}
+
+int holo_set_workspace_limit(holo_handle h, size_t bytes) {
+  if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
+  h->batch_ws_limit = bytes;
   return HOLO_OK;
 }
 

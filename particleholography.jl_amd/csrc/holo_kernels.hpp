@@ -499,8 +499,23 @@ __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
 // Propagation column pass (GS, cu_asm_prop!): forward column FFT, .* H, inverse column FFT.
 // HMODE 0: H generated (FP64-wrapped, natural order); HMODE 1: centred ComplexF32 array.
 // ------------------------------------------------------------------------------------------
-// HMODE 2: natural-order ComplexF32 array (precomputed once per GS call), multiplied as is
-// or conjugated (H(-z) = conj(H(z)) for propagating components).
+// Per-pair parameters of a Gerchberg-Saxton pass; a launch covers up to kMaxJobs pairs of one
+// size (blockIdx.z = entry), the table travelling in the kernel arguments.
+struct GsDesc {
+  float2* B;          // the pair's field (transformed in place)
+  const float2* H;    // natural-order transfer array of this propagation
+  const float* I;     // intensity whose amplitude is imposed by the row pass
+  float2* out;        // the retrieved wavefront (row pass with final = 1)
+  int conj;           // multiply by conj(H)
+  int final;          // row pass of the last iteration: store psi instead of its row FFT
+};
+struct GsTable {
+  int n;
+  GsDesc d[kMaxJobs];
+};
+
+// HMODE 2: natural-order ComplexF32 array from the GS table (precomputed once per GS call),
+// multiplied as is or conjugated (H(-z) = conj(H(z)) for propagating components).
 struct PropArgs {
   const float2* in;
   float2* out;
@@ -510,8 +525,11 @@ struct PropArgs {
 };
 
 template <int E, int L, int HMODE>
-__global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, Geom g) {
+__global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom g) {
   extern __shared__ float2 lds[];
+  const GsDesc& pd = gt.d[HMODE == 2 ? blockIdx.z : 0];
+  const float2* __restrict__ src = HMODE == 2 ? pd.B : a.in;
+  float2* __restrict__ dst = HMODE == 2 ? pd.B : a.out;
   const ColCtx c = col_ctx<E, L>(g, lds);
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
@@ -519,7 +537,7 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, Geom g) {
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e)
-    v[e] = ok ? a.in[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
+    v[e] = ok ? src[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
   fft_run<E, -1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if constexpr (HMODE == 0) {
     const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
@@ -534,28 +552,32 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, Geom g) {
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = cmul(v[e], a.H[centred_index(ci, c.t + T * e, g.nx, g.ny)]);
   } else {
+    const float2* __restrict__ H = pd.H;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const float2 h = a.H[static_cast<size_t>(c.t + T * e) * g.nx + ci];
-      v[e] = a.conj ? cmulconj(v[e], h) : cmul(v[e], h);
+      const float2 h = H[static_cast<size_t>(c.t + T * e) * g.nx + ci];
+      v[e] = pd.conj ? cmulconj(v[e], h) : cmul(v[e], h);
     }
   }
   fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if (ok) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) a.out[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
+    for (int e = 0; e < E; ++e) dst[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
   }
 }
 
-// GS row pass (holofunc.jl:115-125): inverse row FFT (scaled 1/(nx*ny)), amplitude
-// replacement psi <- sqrt(I) * psi/|psi| (|psi| = 0 -> sqrt(I), i.e. angle(0) = 0), then
-// MODE 0: forward row FFT of the result into `out` (first half of the next propagation);
-// MODE 1: store the result (the retrieved wavefront).
-template <int E, int L, int MODE>
-__global__ __launch_bounds__(512) void k_gs_rows(const float2* __restrict__ U,
-                                                 const float* __restrict__ I,
-                                                 float2* __restrict__ out, float sc, Geom g) {
+// GS row pass (holofunc.jl:115-125) for pair blockIdx.z of the table: inverse row FFT (scaled
+// 1/(nx*ny)), amplitude replacement psi <- sqrt(I) * psi/|psi| (|psi| = 0 -> sqrt(I), i.e.
+// angle(0) = 0), then either the forward row FFT of the result back into B (first half of
+// the next propagation) or, in the last iteration, the result itself into `out`.
+template <int E, int L>
+__global__ __launch_bounds__(512) void k_gs_rows(GsTable gt, float sc, Geom g) {
   extern __shared__ float2 lds[];
+  const GsDesc& rd = gt.d[blockIdx.z];
+  const float2* __restrict__ U = rd.B;
+  const float* __restrict__ I = rd.I;
+  const bool final = rd.final != 0;
+  float2* __restrict__ out = final ? rd.out : rd.B;
   const RowCtx c = row_ctx<E, L>(g, lds);
   const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
@@ -578,11 +600,19 @@ __global__ __launch_bounds__(512) void k_gs_rows(const float2* __restrict__ U,
       v[e] = make_float2(A, 0.f);
     }
   }
-  if constexpr (MODE == 0) fft_run<E, -1, L>(v, c.line, 1, c.t, g.px, c.tw);
+  if (!final) fft_run<E, -1, L>(v, c.line, 1, c.t, g.px, c.tw);
   if (c.ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e) out[rowoff + c.t + T * e] = v[e];
   }
+}
+
+// Centred (fftshifted) array -> natural FFT order (ifftshift), for array-backed transfers.
+__global__ void k_to_natural(int nx, int ny, const float2* __restrict__ src, float2* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  dst[static_cast<size_t>(j) * nx + i] = src[centred_index(i, j, nx, ny)];
 }
 
 // ------------------------------------------------------------------------------------------

@@ -36,6 +36,12 @@ class holo_job(ctypes.Structure):
                 ("slices", ctypes.c_int), ("proj", ctypes.c_void_p)]
 
 
+class holo_gs_job(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("holo1", ctypes.c_void_p),
+                ("holo2", ctypes.c_void_p), ("fwd", holo_tf), ("bwd", holo_tf),
+                ("iters", ctypes.c_int), ("out", ctypes.c_void_p)]
+
+
 _vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
 _tfp = ctypes.POINTER(holo_tf)
 
@@ -59,7 +65,9 @@ SIGNATURES = {
     "holo_asm_prop": [_vp, _vp, _i, _i, _vp, _vp, _tfp],
     "holo_fft2": [_vp, _vp, _i, _i, _vp, _vp, _i],
     "holo_reconst_batch": [_vp, _vp, ctypes.POINTER(holo_job), _i],
+    "holo_phase_retrieval_batch": [_vp, _vp, ctypes.POINTER(holo_gs_job), _i],
     "holo_set_tuning_for_size": [_vp, _i, _i, _i, _i, _i, _i, _i],
+    "holo_set_workspace_limit": [_vp, ctypes.c_size_t],
     "holo_get_tuning_for_size": [_vp, _i, _i, _i, ctypes.POINTER(ctypes.c_int)],
     "holo_autotune": [_vp, _vp, _i, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
     "holo_set_profiling": [_vp, _i],
@@ -129,6 +137,9 @@ class Handle:
 
     def set_tuning(self, batch=0, spg_cols=0, spg_rows=0, cols_per_wg=0, rows_per_wg=0):
         check(self.lib.holo_set_tuning(self.ptr, batch, spg_cols, spg_rows, cols_per_wg, rows_per_wg))
+
+    def set_workspace_limit(self, nbytes: int):
+        check(self.lib.holo_set_workspace_limit(self.ptr, int(nbytes)))
 
     def set_profiling(self, enable: bool):
         check(self.lib.holo_set_profiling(self.ptr, 1 if enable else 0))

@@ -37,8 +37,9 @@ class ReconJob:
     proj: Optional[torch.Tensor] = None
 
 
-def reconst_batch(jobs: List[ReconJob], device: int = 0) -> List[torch.Tensor]:
-    """xy min-projections of every job (Float32 (ny, nx) tensors), computed together."""
+def reconst_batch(jobs: List[ReconJob], device: int = 0, handle=None) -> List[torch.Tensor]:
+    """xy min-projections of every job (Float32 (ny, nx) tensors), computed together on the
+    current stream (``handle``: a libholo handle other than the thread's default)."""
     arr = (holo_job * max(1, len(jobs)))()
     keep = []
     for i, j in enumerate(jobs):
@@ -60,7 +61,8 @@ def reconst_batch(jobs: List[ReconJob], device: int = 0) -> List[torch.Tensor]:
         arr[i].slices = int(j.slices)
         arr[i].proj = j.proj.data_ptr()
     stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
-    check(_lib.load().holo_reconst_batch(_lib.handle(device).ptr, stream, arr, len(jobs)))
+    hd = handle if handle is not None else _lib.handle(device)
+    check(_lib.load().holo_reconst_batch(hd.ptr, stream, arr, len(jobs)))
     return [j.proj for j in jobs]
 
 
@@ -113,3 +115,54 @@ def load_tuning(device: int = 0) -> int:
         nx, ny = (int(v) for v in size.split("x"))
         set_tuning_for_size(nx, ny, choice, device)
     return len(entries)
+
+
+@dataclass
+class GSJob:
+    """One Gerchberg-Saxton pair: intensities at two planes separated by +z (holo1 nearer the
+    object), ``iters`` iterations. Transfers are generated from (z, wavlen, dx) unless
+    ``fwd`` / ``bwd`` (centred ComplexF32 arrays, as from cu_transfer) are given."""
+    holo1: torch.Tensor
+    holo2: torch.Tensor
+    z: float
+    iters: int
+    wavlen: float = 0.6328
+    dx: float = 10.0
+    fwd: Optional[torch.Tensor] = None
+    bwd: Optional[torch.Tensor] = None
+    out: Optional[torch.Tensor] = None
+
+
+def phase_retrieval_batch(jobs: List[GSJob], device: int = 0, handle=None) -> List[torch.Tensor]:
+    """Retrieved wavefronts at holo1's plane (ComplexF32 (ny, nx)), computed together on the
+    current stream (``handle``: a libholo handle other than the thread's default)."""
+    from ._lib import holo_gs_job
+
+    arr = (holo_gs_job * max(1, len(jobs)))()
+    keep = []
+    for i, j in enumerate(jobs):
+        h1, h2 = j.holo1.contiguous(), j.holo2.contiguous()
+        if h1.dtype != torch.float32 or h2.dtype != torch.float32 or h1.shape != h2.shape:
+            raise AssertionError(f"job {i}: holograms must be float32 tensors of one shape")
+        if not h1.is_cuda:
+            raise AssertionError(f"job {i}: arrays must be device (GPU) tensors")
+        ny, nx = h1.shape
+        if j.out is None:
+            j.out = torch.empty((ny, nx), dtype=torch.complex64, device=h1.device)
+        keep += [h1, h2]
+        arr[i].nx, arr[i].ny = nx, ny
+        arr[i].holo1, arr[i].holo2 = h1.data_ptr(), h2.data_ptr()
+        if j.fwd is not None:
+            f, b = j.fwd.contiguous(), j.bwd.contiguous()
+            keep += [f, b]
+            arr[i].fwd = holo_tf(f.data_ptr(), 0.0, 0.0, 0.0, 0)
+            arr[i].bwd = holo_tf(b.data_ptr(), 0.0, 0.0, 0.0, 0)
+        else:
+            arr[i].fwd = holo_tf(None, float(j.z), float(j.wavlen), float(j.dx), 1)
+            arr[i].bwd = holo_tf(None, -float(j.z), float(j.wavlen), float(j.dx), 1)
+        arr[i].iters = int(j.iters)
+        arr[i].out = j.out.data_ptr()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    hd = handle if handle is not None else _lib.handle(device)
+    check(_lib.load().holo_phase_retrieval_batch(hd.ptr, stream, arr, len(jobs)))
+    return [j.out for j in jobs]

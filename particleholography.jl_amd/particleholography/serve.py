@@ -1,0 +1,277 @@
+"""Continuous-batching service for hologram reconstruction and phase-retrieval requests.
+
+A request is one hologram to reconstruct (xy min-projection over its slices), or a two-plane
+Gerchberg-Saxton pair whose retrieved wavefront is then reconstructed (the reference's
+workflow: cu_phase_retrieval_holo followed by cu_get_reconst_xyprojection, pr.md:95-109).
+
+The serving loop (one host thread):
+
+* requests arrive over time (their ``arrival`` offsets) into a FIFO;
+* admission: the oldest waiting requests are admitted while their device-memory footprint
+  (inputs, outputs, retrieved wavefronts) fits the budget left by the in-flight work and the
+  slots' reserved workspaces, up to ``max_batch`` per launch;
+* an admitted group runs on a free slot -- a default-priority stream with its own libholo
+  handle (own workspace, plans and tuning) -- as one batched phase retrieval (its GS requests)
+  and one batched reconstruction, so groups admitted while others run overlap on the device;
+* completion is detected by polling an event per group; the projection is copied back, the
+  footprint released and the latency (completion - arrival) recorded.
+
+A request's result does not depend on what it is batched with: the batched entry points
+compute every hologram exactly as the single-hologram calls do.
+"""
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .batch import GSJob, ReconJob, phase_retrieval_batch, reconst_batch
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float                 # seconds after the start of the stream
+    holo: torch.Tensor             # Float32 (ny, nx), host (pinned) or device
+    z_front: float
+    z_dz: float
+    slices: int
+    holo2: Optional[torch.Tensor] = None   # GS requests: the second plane
+    gs_z: float = 0.0
+    gs_iters: int = 0
+    wavlen: float = 0.6328
+    dx: float = 10.0
+    # filled by the server
+    proj: Optional[torch.Tensor] = None
+    t_admit: float = math.nan
+    t_done: float = math.nan
+
+    @property
+    def is_gs(self) -> bool:
+        return self.holo2 is not None
+
+    def footprint(self) -> int:
+        """Device bytes held while in flight: inputs, projection, GS wavefront."""
+        ny, nx = self.holo.shape
+        px = nx * ny
+        b = 4 * px + 4 * px
+        if self.is_gs:
+            b += 4 * px + 8 * px
+        return b
+
+    def work_bytes(self) -> int:
+        """Algorithmic HBM bytes: 32 B/px per slice, 72 B/px per GS iteration."""
+        ny, nx = self.holo.shape
+        return nx * ny * (32 * self.slices + 72 * self.gs_iters)
+
+
+@dataclass
+class _Slot:
+    stream: torch.cuda.Stream
+    handle: _lib.Handle
+    busy: bool = False
+
+
+@dataclass
+class _Group:
+    slot: _Slot
+    reqs: List[Request]
+    event: torch.cuda.Event
+    bytes: int
+    outs: list = field(default_factory=list)
+
+
+def select_batch(waiting: deque, used: int, budget: int, max_batch: int, others_in_flight: bool) -> list:
+    """Admission: the oldest waiting requests, in order, while their footprints fit the budget
+    left by the work in flight, up to ``max_batch``. A request larger than the whole budget is
+    still admitted alone when nothing else is in flight (it could never fit otherwise)."""
+    batch, fp = [], 0
+    while waiting and len(batch) < max_batch:
+        f = waiting[0].footprint()
+        if used + fp + f > budget and (batch or others_in_flight):
+            break
+        batch.append(waiting.popleft())
+        fp += f
+    return batch
+
+
+class Server:
+    def __init__(self, device: int = 0, n_slots: int = 4, max_batch: int = 32,
+                 hbm_budget: int = 32 << 30, workspace_per_slot: int = 2 << 30):
+        self.device = device
+        self.dev = torch.device(f"cuda:{device}")
+        self.max_batch = max_batch
+        self.budget = hbm_budget - n_slots * workspace_per_slot
+        if self.budget <= 0:
+            raise ValueError("HBM budget smaller than the slots' workspaces")
+        self.slots = [_Slot(torch.cuda.Stream(device=self.dev), _lib.Handle(device)) for _ in range(n_slots)]
+        for s in self.slots:
+            s.handle.set_workspace_limit(workspace_per_slot)
+
+    def _launch(self, slot: _Slot, reqs: List[Request], now: float) -> _Group:
+        with torch.cuda.stream(slot.stream):
+            dev_in = []
+            for r in reqs:
+                h1 = r.holo.to(self.dev, non_blocking=True)
+                h2 = r.holo2.to(self.dev, non_blocking=True) if r.is_gs else None
+                dev_in.append((h1, h2))
+            gs = [(i, r) for i, r in enumerate(reqs) if r.is_gs]
+            wfs = {}
+            if gs:
+                outs = phase_retrieval_batch([GSJob(dev_in[i][0], dev_in[i][1], r.gs_z, r.gs_iters, r.wavlen, r.dx)
+                                              for i, r in gs], self.device, slot.handle)
+                wfs = {i: o for (i, _), o in zip(gs, outs)}
+            jobs = [ReconJob(wfs.get(i, dev_in[i][0]), r.z_front, r.z_dz, r.slices, r.wavlen, r.dx)
+                    for i, r in enumerate(reqs)]
+            projs = reconst_batch(jobs, self.device, slot.handle)
+            host = [p.to("cpu", non_blocking=True) for p in projs]
+            ev = torch.cuda.Event()
+            ev.record(slot.stream)
+        for r in reqs:
+            r.t_admit = now
+        g = _Group(slot, reqs, ev, sum(r.footprint() for r in reqs))
+        g.outs = [dev_in, projs, wfs, host]  # keep buffers alive until the group completes
+        return g
+
+    def run(self, requests: List[Request], spin: float = 50e-6) -> dict:
+        """Serves the stream in real time (arrival offsets from now); returns statistics."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting: deque = deque()
+        inflight: List[_Group] = []
+        used = 0
+        t0 = time.perf_counter()
+        groups = 0
+        while pending or waiting or inflight:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            for g in list(inflight):
+                if g.event.query():
+                    done = time.perf_counter() - t0
+                    for r, hp in zip(g.reqs, g.outs[3]):
+                        r.proj = hp
+                        r.t_done = done
+                    used -= g.bytes
+                    g.slot.busy = False
+                    inflight.remove(g)
+            free = [s for s in self.slots if not s.busy]
+            if waiting and free:
+                batch = select_batch(waiting, used, self.budget, self.max_batch, bool(inflight))
+                fp = sum(r.footprint() for r in batch)
+                if batch:
+                    slot = free[0]
+                    slot.busy = True
+                    inflight.append(self._launch(slot, batch, now))
+                    used += fp
+                    groups += 1
+                    continue
+            if not (waiting and free):
+                if pending and not inflight and not waiting:
+                    time.sleep(max(0.0, min(pending[0].arrival - now, 0.002)))
+                else:
+                    time.sleep(spin)
+        makespan = time.perf_counter() - t0
+        return summarize(requests, makespan, groups)
+
+
+def summarize(requests: List[Request], makespan: float, groups: int = 0) -> dict:
+    lat = np.array([r.t_done - r.arrival for r in requests])
+    slices = sum(r.slices for r in requests)
+    eq1024 = sum(r.slices * r.holo.shape[0] * r.holo.shape[1] for r in requests) / (1024 * 1024)
+    work = sum(r.work_bytes() for r in requests)
+    return {"requests": len(requests), "groups": groups, "makespan_s": makespan,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3), "slices_per_s": slices / makespan,
+            "slices_1024eq_per_s": eq1024 / makespan, "algorithmic_GBs": work / makespan / 1e9}
+
+
+def plain_one_at_a_time(requests: List[Request], device: int = 0) -> dict:
+    """The comparator: every request alone, in arrival order, with the single-hologram entry
+    points on one stream (H2D, phase retrieval, reconstruction, D2H), no overlap."""
+    dev = torch.device(f"cuda:{device}")
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    for r in requests:
+        h1 = r.holo.to(dev)
+        if r.is_gs:
+            src = _single_gs(h1, r.holo2.to(dev), r)
+        else:
+            src = h1
+        proj = _single_recon(src, r)
+        r.proj = proj.cpu()
+        r.t_done = time.perf_counter() - t0
+    makespan = time.perf_counter() - t0
+    return summarize_closed(requests, makespan)
+
+
+def _single_gs(h1, h2, r):
+    import ctypes
+    out = torch.empty(h1.shape, dtype=torch.complex64, device=h1.device)
+    ny, nx = h1.shape
+    f = _lib.holo_tf(None, r.gs_z, r.wavlen, r.dx, 1)
+    b = _lib.holo_tf(None, -r.gs_z, r.wavlen, r.dx, 1)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(_lib.load().holo_phase_retrieval(_lib.handle(0).ptr, st, nx, ny, ctypes.c_void_p(h1.data_ptr()),
+                                                ctypes.c_void_p(h2.data_ptr()), ctypes.byref(f), ctypes.byref(b),
+                                                r.gs_iters, ctypes.c_void_p(out.data_ptr()), 1))
+    return out
+
+
+def _single_recon(src, r):
+    import ctypes
+    ny, nx = src.shape
+    proj = torch.empty((ny, nx), dtype=torch.float32, device=src.device)
+    f = _lib.holo_tf(None, r.z_front, r.wavlen, r.dx, 1)
+    d = _lib.holo_tf(None, r.z_dz, r.wavlen, r.dx, 1)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = _lib.load()
+    fn = lib.holo_reconst_xyprojection_holo if src.dtype == torch.float32 else lib.holo_reconst_xyprojection
+    _lib.check(fn(_lib.handle(0).ptr, st, nx, ny, ctypes.c_void_p(src.data_ptr()), ctypes.byref(f),
+                  ctypes.byref(d), r.slices, ctypes.c_void_p(proj.data_ptr())))
+    return proj
+
+
+def summarize_closed(requests: List[Request], makespan: float) -> dict:
+    slices = sum(r.slices for r in requests)
+    work = sum(r.work_bytes() for r in requests)
+    return {"requests": len(requests), "makespan_s": makespan, "slices_per_s": slices / makespan,
+            "algorithmic_GBs": work / makespan / 1e9}
+
+
+SIZES = [(512, 512), (768, 768), (1024, 1024), (1280, 1024), (2048, 2048)]
+
+
+def make_stream(n: int, rate: float, seed: int = 105, pool: int = 3, gs_fraction: float = 0.2,
+                slices=(128, 1024), gs_iters: int = 50, device: int = 0, pin: bool = True,
+                sizes=SIZES) -> List[Request]:
+    """A seeded Poisson stream of ``n`` requests at ``rate`` requests/s. Holograms come from a
+    pool of ``pool`` seeded scenes per size (formed on the device, copied to pinned host
+    memory, as a camera would deliver them); each request draws its own size, slice count
+    (uniform in ``slices``), depth range and, for GS requests, 50 iterations at 2000 um."""
+    from . import synth
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scenes = {}
+    for si, (nx, ny) in enumerate(sizes):
+        for k in range(pool):
+            s = seed * 1000 + si * 10 + k
+            h1 = synth.hologram_device(s, 300, nx, ny, zmin=1e4, zmax=1e5, device=device)
+            h2 = synth.hologram_device(s + 500, 300, nx, ny, zmin=1.2e4, zmax=1.02e5, device=device)
+            scenes[(si, k)] = tuple(t.cpu().pin_memory() if pin else t.cpu() for t in (h1, h2))
+    t, reqs = 0.0, []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        si = int(rng.integers(len(sizes)))
+        k = int(rng.integers(pool))
+        s = int(rng.integers(slices[0], slices[1] + 1))
+        h1, h2 = scenes[(si, k)]
+        gs = rng.random() < gs_fraction
+        reqs.append(Request(i, t, h1, -1e4, -(2e5 - 1e4) / s, s, holo2=h2 if gs else None,
+                            gs_z=2000.0 if gs else 0.0, gs_iters=gs_iters if gs else 0))
+    return reqs

@@ -132,3 +132,46 @@ def gs_pair(seed: int, ndisks: int, n: int, separation: float, zmin: float = 1e4
     psi2 = _propagate(psi1, separation, kzc, evan)
     rng = np.random.Generator(np.random.PCG64(seed + 0x9E3779B9))
     return _record(psi1, rng, noise), _record(psi2, rng, noise)
+
+
+def hologram_device(seed: int, ndisks: int, nx: int, ny: int | None = None, zmin: float = 1e4,
+                    zmax: float = 1e5, wavlen: float = 0.6328, dx: float = 10.0, noise: float = 0.01,
+                    max_planes: int = 16, device: int = 0):
+    """The same scene model as ``hologram`` formed on the GPU: disk masks are applied on the
+    device and the field is carried between depth planes by libholo's angular-spectrum
+    propagation in complex FP32 (FP64-wrapped transfer phase). For benchmark inputs in bulk;
+    the tests' parity cases use the FP64 ``hologram``. Returns a Float32 (ny, nx) device tensor."""
+    import ctypes
+
+    import torch
+
+    from . import _lib
+
+    ny = nx if ny is None else ny
+    x, y, z, diam = disk_scene(seed, ndisks, nx, ny, zmin, zmax, dx=dx)
+    dev = torch.device(f"cuda:{device}")
+    psi = torch.ones((ny, nx), dtype=torch.complex64, device=dev)
+    lib, h = _lib.load(), _lib.handle(device)
+    st = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+    def prop(dist):
+        t = _lib.holo_tf(None, float(dist), float(wavlen), float(dx), 1)
+        _lib.check(lib.holo_asm_prop(h.ptr, st, nx, ny, ctypes.c_void_p(psi.data_ptr()),
+                                     ctypes.c_void_p(psi.data_ptr()), ctypes.byref(t)))
+
+    zcur = None
+    for zp, members in _planes(z, zmin, zmax, max_planes):
+        if zcur is not None:
+            prop(zcur - zp)
+        mask = np.ones((ny, nx), dtype=np.float32)
+        for i in members:
+            _apply_disk(mask, x[i], y[i], diam[i] / 2.0, dx)
+        psi *= torch.from_numpy(mask).to(dev)
+        zcur = zp
+    if zcur is not None:
+        prop(zcur)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed) + 0x9E3779B9)
+    inten = psi.real ** 2 + psi.imag ** 2
+    inten = inten + noise * torch.randn((ny, nx), generator=gen, device=dev, dtype=torch.float32)
+    return inten.clamp_min(0.0).contiguous()

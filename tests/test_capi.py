@@ -47,14 +47,33 @@ def test_supported_lengths_host_only():
     assert lib.holo_version() >= 1
 
 
-def test_holo_tf_struct_layout():
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """ctypes mirrors of holo_tf / holo_job / holo_gs_job against gcc's layout of the header."""
+    import subprocess
+
     from particleholography import _lib
 
-    assert ctypes.sizeof(_lib.holo_tf) == 40
-    assert _lib.holo_tf.z.offset == 8 and _lib.holo_tf.generated.offset == 32
-    assert ctypes.sizeof(_lib.holo_job) == 120
-    assert _lib.holo_job.front.offset == 24 and _lib.holo_job.slices.offset == 104
-    assert _lib.holo_job.proj.offset == 112
+    fields = {"holo_tf": ["data", "z", "wavlen", "dx", "generated"],
+              "holo_job": ["nx", "ny", "src", "src_real", "front", "dz", "slices", "proj"],
+              "holo_gs_job": ["nx", "ny", "holo1", "holo2", "fwd", "bwd", "iters", "out"]}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "holo.h"', "int main(void) {"]
+    for st, fs in fields.items():
+        lines.append(f'  printf("{st} %zu", sizeof({st}));')
+        for f in fs:
+            lines.append(f'  printf(" %zu", offsetof({st}, {f}));')
+        lines.append('  printf("\\n");')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(HEADER.parent), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        name, size, *offs = line.split()
+        cls = getattr(_lib, name)
+        assert ctypes.sizeof(cls) == int(size), name
+        for f, o in zip(fields[name], offs):
+            assert getattr(cls, f).offset == int(o), (name, f)
 
 
 def test_mirror_validates_sizes_before_device_work():

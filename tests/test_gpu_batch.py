@@ -96,3 +96,54 @@ def test_autotune_keeps_results_correct(gpu):
         assert asm.rel_l2(got, want) < 1e-4
     finally:
         batch.set_tuning_for_size(n, n, {})
+
+
+def _gs_pair(n, seed, z=2000.0):
+    rng = np.random.default_rng(seed)
+    psi = np.exp(1j * 0.7 * rng.standard_normal((n, n))) * (1 + 0.3 * rng.random((n, n)))
+    i1 = (np.abs(psi) ** 2).astype(np.float32)
+    i2 = (np.abs(asm.asm_prop(psi, z, LAM, DX)) ** 2).astype(np.float32)
+    return i1, i2
+
+
+def test_gs_batch_equals_single_calls(gpu):
+    import particleholography as ph
+    from particleholography.batch import GSJob, phase_retrieval_batch
+
+    specs = [(64, 5, 2000.0), (128, 12, 1500.0), (64, 1, 3000.0), (128, 7, 2000.0), (96, 0, 2000.0), (64, 9, 2500.0)]
+    jobs, ins = [], []
+    for k, (n, it, z) in enumerate(specs):
+        i1, i2 = _gs_pair(n, 40 + k, z)
+        t1, t2 = torch.from_numpy(i1).to(gpu), torch.from_numpy(i2).to(gpu)
+        ins.append((t1, t2, n, it, z))
+        jobs.append(GSJob(t1, t2, z, it, LAM, DX))
+    # one pair with array-backed transfers in the reference's arithmetic
+    n = 64
+    dsq = ph.cu_transfer_sqrt_arr(n, LAM, DX)
+    hf, hb = ph.cu_transfer(1800.0, n, LAM, dsq, precision="reference"), ph.cu_transfer(-1800.0, n, LAM, dsq, precision="reference")
+    a1, a2 = _gs_pair(n, 99, 1800.0)
+    ta1, ta2 = torch.from_numpy(a1).to(gpu), torch.from_numpy(a2).to(gpu)
+    jobs.append(GSJob(ta1, ta2, 1800.0, 6, fwd=hf.data, bwd=hb.data))
+    outs = phase_retrieval_batch(jobs)
+    for (t1, t2, m, it, z), out in zip(ins, outs):
+        dsq_m = ph.cu_transfer_sqrt_arr(m, LAM, DX)
+        want = ph.cu_phase_retrieval_holo(t1, t2, ph.cu_transfer(z, m, LAM, dsq_m), ph.cu_transfer(-z, m, LAM, dsq_m), it, m)
+        np.testing.assert_array_equal(out.cpu().numpy(), want.data.cpu().numpy())
+    want = ph.cu_phase_retrieval_holo(ta1, ta2, hf, hb, 6, n)
+    np.testing.assert_array_equal(outs[-1].cpu().numpy(), want.data.cpu().numpy())
+    ref = asm.phase_retrieval(a1, a2, 6, hf.data.cpu().numpy(), hb.data.cpu().numpy())
+    assert asm.weighted_phase_rms(outs[-1].cpu().numpy(), ref) < 1e-3
+
+
+def test_gs_batch_many_pairs_vs_oracle(gpu):
+    from particleholography.batch import GSJob, phase_retrieval_batch
+
+    jobs, pairs = [], []
+    for k in range(34):  # two launches per pass (32 pairs max per launch)
+        i1, i2 = _gs_pair(32, 200 + k)
+        pairs.append((i1, i2))
+        jobs.append(GSJob(torch.from_numpy(i1).to(gpu), torch.from_numpy(i2).to(gpu), 2000.0, 3))
+    outs = phase_retrieval_batch(jobs)
+    for (i1, i2), out in zip(pairs, outs):
+        ref = asm.phase_retrieval(i1, i2, 3, z=2000.0, wavlen=LAM, dx=DX)
+        assert asm.weighted_phase_rms(out.cpu().numpy(), ref) < 1e-3
