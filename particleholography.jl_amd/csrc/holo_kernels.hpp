@@ -154,12 +154,22 @@ struct ColCtx {
   float2* col;
   const float2* tw;
 };
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2) by linear
+// workgroup id. Column tiles are renumbered so that each XCD walks a contiguous range of
+// tiles: neighbouring tiles, which share cache lines when W*8 bytes is below a line, meet in
+// the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n / kXcds;
+  return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
+}
+
 template <int E, int L>
 __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   ColCtx c;
   c.w = threadIdx.x & (g.W - 1);
   c.t = threadIdx.x >> g.wshift;
-  c.i = blockIdx.x * g.W + c.w;
+  c.i = xcd_tile(blockIdx.x, gridDim.x) * g.W + c.w;
   c.col = lds + c.w;
   float2* tw = lds + g.W * lds_padded_len(g.ny);
   stage_twiddles(tw, g.twy, g.py.ntw);
