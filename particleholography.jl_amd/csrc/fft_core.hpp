@@ -241,6 +241,22 @@ template <int DIR>
 struct DFT<16, DIR> {
   __device__ __forceinline__ static void run(float2* x) { dft_composite<4, 4, DIR>(x); }
 };
+template <int DIR>
+struct DFT<6, DIR> {
+  __device__ __forceinline__ static void run(float2* x) { dft_composite<2, 3, DIR>(x); }
+};
+template <int DIR>
+struct DFT<10, DIR> {
+  __device__ __forceinline__ static void run(float2* x) { dft_composite<2, 5, DIR>(x); }
+};
+template <int DIR>
+struct DFT<12, DIR> {
+  __device__ __forceinline__ static void run(float2* x) { dft_composite<4, 3, DIR>(x); }
+};
+template <int DIR>
+struct DFT<20, DIR> {
+  __device__ __forceinline__ static void run(float2* x) { dft_composite<4, 5, DIR>(x); }
+};
 
 // ------------------------------------------------------------------------------------------
 // LDS addressing: one padding slot every 16 elements breaks the power-of-two strides of the
@@ -317,8 +333,12 @@ __device__ __forceinline__ void fft_line(float2 (&v)[E], float2* __restrict__ ld
       case 3: fft_stage_if<3, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
       case 4: fft_stage_if<4, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
       case 5: fft_stage_if<5, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 6: fft_stage_if<6, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
       case 8: fft_stage_if<8, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 10: fft_stage_if<10, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 12: fft_stage_if<12, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
       case 16: fft_stage_if<16, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 20: fft_stage_if<20, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
       default: break;
     }
   }
@@ -329,6 +349,11 @@ __device__ __forceinline__ void fft_line(float2 (&v)[E], float2* __restrict__ ld
 // as the host planner (largest radix dividing both E and the remaining length, first), with
 // every radix, span, stride and table offset a constant.
 // ------------------------------------------------------------------------------------------
+// Radices in the order the planner tries them (the largest dividing both E and the remaining
+// length first); the host planner uses the same list.
+constexpr int kNumRadices = 10;
+constexpr int kRadixOrder[kNumRadices] = {20, 16, 12, 10, 8, 6, 5, 4, 3, 2};
+
 struct CPlanData {
   int ns;
   int R[kMaxStages];
@@ -339,11 +364,11 @@ struct CPlanData {
 
 constexpr CPlanData make_cplan(int L, int E) {
   CPlanData d{};
-  const int radices[6] = {16, 8, 5, 4, 3, 2};
+  const int* radices = kRadixOrder;
   int rem = L, span = 1, off = 0;
   while (rem > 1 && d.ns < kMaxStages) {
     int R = 0;
-    for (int i = 0; i < 6 && R == 0; ++i)
+    for (int i = 0; i < kNumRadices && R == 0; ++i)
       if (E % radices[i] == 0 && rem % radices[i] == 0) R = radices[i];
     if (R == 0) break;
     d.R[d.ns] = R;

@@ -197,16 +197,18 @@ int get_plan(holo_context* h, int L, const Plan1D** out) {
   p.lp.L = L;
   p.lp.T = L / E;
   p.lp.ns = 0;
-  static const int kRadices[] = {16, 8, 5, 4, 3, 2};
+  const int* kRadices = kRadixOrder;
   int rem = L, span = 1, twoff = 0;
   std::vector<float2> tw;
   while (rem > 1) {
     int R = 0;
-    for (int r : kRadices)
+    for (int ri = 0; ri < kNumRadices; ++ri) {
+      const int r = kRadices[ri];
       if (E % r == 0 && rem % r == 0) {
         R = r;
         break;
       }
+    }
     if (R == 0 || p.lp.ns >= kMaxStages) return fail(HOLO_ERR_SIZE, "no radix plan for length " + std::to_string(L));
     StagePlan& sp = p.lp.st[p.lp.ns++];
     sp.R = R;
