@@ -163,6 +163,7 @@ def main():
         extras["plain_path_slices_per_s"] = round(plain, 1)
         extras["speedup_vs_plain"] = round(slices_per_s / plain, 2)
         extras["gs_1024"] = bench_gs(ph, synth, torch, dev, d_sqr)
+        extras["lpf_1024"] = bench_lpf(ph, torch, holo)
         extras["batched_64"] = bench_batched(torch, dev, args.batch_holograms)
         extras["serving"] = bench_serving(torch, args.serve_requests)
 
@@ -204,6 +205,18 @@ def bench_gs(ph, synth, torch, dev, d_sqr):
     tw, _ = timed_steps(run, 3, 1, torch)
     return {"iterations_per_s": round(300 / gw, 1), "gs_algorithmic_GBs": round(72.0 * N * N * 300 / gw / 1e9, 1),
             "pipeline_ms": round(tw / 3 * 1e3, 3), "config": "1024^2, 100 iterations at 2000 um, then 512 slices"}
+
+
+def bench_lpf(ph, torch, holo):
+    """Super-Gaussian low-pass filter on the 1024^2 wavefront, applied in place (the tutorial's
+    step before reconstruction). Algorithmic bytes per application: 52 per pixel (row pass
+    8 in + 8 out, column pass 8 + 4 filter in + 8 out, inverse row pass 8 in + 8 out)."""
+    wf = ph.cu_gabor_wavefront(holo)
+    lpf = ph.cu_super_gaussian_filter(Z0 + DZ * SLICES, LAM, N, DX)
+    reps = 200
+    _, sec = timed_steps(lambda: ph.cu_apply_low_pass_filter_(wf, lpf), reps, 10, torch)
+    return {"apply_us": round(sec / reps * 1e6, 2), "algorithmic_GBs": round(52.0 * N * N * reps / sec / 1e9, 1),
+            "config": "1024^2 ComplexF32 wavefront, order-3 super-Gaussian cutoff for 2.1e5 um"}
 
 
 def bench_batched(torch, dev, nholo):

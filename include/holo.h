@@ -147,6 +147,22 @@ int holo_asm_prop(holo_handle h, void* stream, int nx, int ny, void* out, const 
 int holo_fft2(holo_handle h, void* stream, int nx, int ny, const void* in, void* out,
               int inverse);
 
+/* ---- low-pass filters: cu_rectangle_filter, cu_super_gaussian_filter (frequency_filters.jl:31-74)
+ *      and cu_apply_low_pass_filter(!) (frequency_filters.jl:88-107) ---------------------- */
+enum holo_lpf_kind {
+  HOLO_LPF_RECTANGLE = 0,      /* 1 inside the cutoff circle, 0 outside */
+  HOLO_LPF_SUPER_GAUSSIAN = 1  /* exp(-(r/sigma)^6 / 2), one half at the cutoff radius */
+};
+/* Centred n x n Float32 filter for objects up to prop_dist away (cutoff radius
+ * n^2 dx^2 / (wavlen sqrt(4 prop_dist^2 + n^2 dx^2)) samples, Fugal 2009 Eq. 13-15), computed
+ * in FP64 with the reference's operation order and centring. */
+int holo_lowpass_filter(holo_handle h, void* stream, int kind, double prop_dist, double wavlen,
+                        int n, double dx, float* out);
+/* out = IFFT2(ifftshift(lpf .* fftshift(FFT2(in)))) for a centred Float32 nx*ny filter;
+ * out may alias in (cu_apply_low_pass_filter! passes the same buffer). */
+int holo_apply_low_pass_filter(holo_handle h, void* stream, int nx, int ny, void* out,
+                               const void* in, const float* lpf);
+
 /* ---- batched reconstruction (no counterpart in the reference: its functions take one
  *      hologram; this runs many cu_get_reconst_xyprojection calls, holofunc.jl:248-266, together) */
 typedef struct holo_job {

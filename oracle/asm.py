@@ -279,6 +279,57 @@ def gs_error(psi1, holo2, h_fwd_nat) -> float:
 # --------------------------------------------------------------------------------------
 # metrics
 # --------------------------------------------------------------------------------------
+# --------------------------------------------------------------------------------------
+# Low-pass filters  (frequency_filters.jl:5-107)
+# --------------------------------------------------------------------------------------
+def lpf_cutoff(prop_dist: float, wavlen: float, imglen: int, dx: float) -> float:
+    """Cutoff radius in frequency samples, FP64 in the reference's order
+    (frequency_filters.jl:33, 68): 1/wavlen * imglen^2 * dx^2 / sqrt(4 z^2 + imglen^2 dx^2)."""
+    n2 = float(imglen * imglen)
+    p2 = dx * dx
+    return 1.0 / wavlen * n2 * p2 / np.sqrt(4.0 * (prop_dist * prop_dist) + n2 * p2)
+
+
+def super_gaussian_sigma(prop_dist: float, wavlen: float, imglen: int, dx: float) -> float:
+    """sigma_x = maxi / (imglen * dx) / (2 log 2)^(1/6)  (frequency_filters.jl:69)."""
+    return lpf_cutoff(prop_dist, wavlen, imglen, dx) / (imglen * dx) / (2.0 * np.log(2.0)) ** (1 / 6)
+
+
+def rectangle_filter(prop_dist: float, wavlen: float, imglen: int, dx: float) -> np.ndarray:
+    """frequency_filters.jl:5-15: arr[y, x] = 0 where (x - n/2 + 1)^2 + (y - n/2)^2 >= maxi^2,
+    1 elsewhere (1-based x = Julia dim 2, y = Julia dim 1). Shape (n, n), Float32."""
+    maxi = lpf_cutoff(prop_dist, wavlen, imglen, dx)
+    idx = np.arange(1, imglen + 1, dtype=np.float64)
+    hx = idx - imglen / 2 + 1.0   # x: numpy row index + 1
+    hy = idx - imglen / 2         # y: numpy column index + 1
+    r2 = hx[:, None] * hx[:, None] + hy[None, :] * hy[None, :]
+    return np.where(r2 >= maxi * maxi, 0.0, 1.0).astype(np.float32)
+
+
+def super_gaussian_at(x, y, imglen: int, dx: float, sigma: float):
+    """exp(-1/2 * (((x - n/2 + 1)/n/dx/sigma)^2 + ((y - n/2 + 1)/n/dx/sigma)^2)^3) at 1-based
+    (possibly fractional) Julia coordinates x (dim 2), y (dim 1), FP64
+    (frequency_filters.jl:46)."""
+    ax = (np.asarray(x, np.float64) - imglen / 2 + 1.0) / imglen / dx / sigma
+    ay = (np.asarray(y, np.float64) - imglen / 2 + 1.0) / imglen / dx / sigma
+    s = ax * ax + ay * ay
+    return np.exp(-0.5 * (s * s * s))
+
+
+def super_gaussian_filter(prop_dist: float, wavlen: float, imglen: int, dx: float) -> np.ndarray:
+    """frequency_filters.jl:40-50, 66-74 on the integer grid, rounded to Float32. Shape (n, n)."""
+    sigma = super_gaussian_sigma(prop_dist, wavlen, imglen, dx)
+    idx = np.arange(1, imglen + 1, dtype=np.float64)
+    return super_gaussian_at(idx[:, None], idx[None, :], imglen, dx, sigma).astype(np.float32)
+
+
+def apply_low_pass_filter(psi: np.ndarray, lpf: np.ndarray) -> np.ndarray:
+    """ifft(ifftshift(lpf .* fftshift(fft(psi))))  (frequency_filters.jl:89, 106), FP64 FFTs
+    with the reference's explicit shifts."""
+    spec = np.fft.fftshift(np.fft.fft2(np.asarray(psi, np.complex128)))
+    return np.fft.ifft2(np.fft.ifftshift(np.asarray(lpf, np.float64) * spec))
+
+
 def rel_l2(a, b) -> float:
     a = np.asarray(a, dtype=np.complex128 if np.iscomplexobj(a) else np.float64)
     b = np.asarray(b, dtype=a.dtype)
@@ -308,4 +359,11 @@ PARITY = {
     "pad2/padded": "cu_2d_pad known-answer test (runtests.jl:362-368)",
     "phase_retrieval": "plane-wave fixed point, |psi1| == sqrt(I1), error-reduction non-increase, "
                        "brute-force DFT GS iteration at 8^2",
+    "lpf_cutoff/rectangle_filter": "cutoff = n dx sin(theta)/wavlen with tan(theta) = (n dx/2)/z "
+                                   "(Fugal 2009 geometry), pass-band area ~ pi maxi^2, disk centred "
+                                   "one sample apart on the two axes, hand-checked 8x8 mask",
+    "super_gaussian_filter": "1 at the centre, exactly 1/2 at radius maxi on both axes, radial "
+                             "monotonicity, x<->y symmetry",
+    "apply_low_pass_filter": "all-ones filter = identity, DC-only filter = mean, single-frequency "
+                             "eigenfunction scaled by its filter value, O(N^4) brute-force DFT",
 }

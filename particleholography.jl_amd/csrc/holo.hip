@@ -607,18 +607,21 @@ int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan
 
 // One propagation column pass over a single plane: in -> FFT_col -> .* H(t) -> IFFT_col -> out
 // (H generated per element, or read from the centred array; cu_asm_prop!).
+// Column pass of a propagation (transfer `t`) or, when `lpf` is given, of a low-pass filter.
 int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, float2* out,
-                     hipStream_t st) {
+                     hipStream_t st, const float* lpf = nullptr) {
   PropArgs pa{};
   pa.in = in;
   pa.out = out;
-  const int mode = t->generated ? 0 : 1;
+  const int mode = lpf ? 3 : t->generated ? 0 : 1;
   if (mode == 0) {
     pa.gp.wavlen = t->wavlen;
     pa.gp.dx = t->dx;
     pa.gp.zl0 = t->z / t->wavlen;
-  } else {
+  } else if (mode == 1) {
     pa.H = static_cast<const float2*>(t->data);
+  } else {
+    pa.lpf = lpf;
   }
   GsTable gt{};
   const size_t lds = col_lds(g);
@@ -630,7 +633,8 @@ int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, 
       hipLaunchKernelGGL(kern, col_grid(g), col_block(g), lds, st, pa, gt, g);
     };
     if (mode == 0) launch(k_prop_cols<E, L, 0>);
-    else launch(k_prop_cols<E, L, 1>);
+    else if (mode == 1) launch(k_prop_cols<E, L, 1>);
+    else launch(k_prop_cols<E, L, 3>);
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
   return check_launch();
@@ -877,6 +881,45 @@ int holo_asm_prop(holo_handle h, void* stream, int nx, int ny, void* out, const 
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
   HOLO_TRY(launch_rows_fft(g, Ex, -1, 0, in, 0, B, 0, 1.0f, 1, st));
   HOLO_TRY(launch_prop_cols(g, Ey, tf, B, B, st));
+  return launch_rows_fft(g, Ex, +1, 0, B, 0, static_cast<float2*>(out), 0, sc, 1, st);
+}
+
+int holo_lowpass_filter(holo_handle h, void* stream, int kind, double prop_dist, double wavlen,
+                        int n, double dx, float* out) {
+#pragma clang fp contract(off)
+  if (!h || !out) return fail(HOLO_ERR_ARG, "NULL handle or output");
+  if (n < 1) return fail(HOLO_ERR_ARG, "imglen must be positive");
+  if (kind != HOLO_LPF_RECTANGLE && kind != HOLO_LPF_SUPER_GAUSSIAN)
+    return fail(HOLO_ERR_ARG, "kind must be HOLO_LPF_RECTANGLE or HOLO_LPF_SUPER_GAUSSIAN");
+  // cutoff radius in frequency samples (frequency_filters.jl:33, 68), FP64 in that order
+  const double n2 = static_cast<double>(static_cast<long long>(n) * n);
+  const double p2 = dx * dx;
+  const double maxi = 1.0 / wavlen * n2 * p2 / sqrt(4.0 * (prop_dist * prop_dist) + n2 * p2);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid((n + 255) / 256, n), block(256);
+  if (kind == HOLO_LPF_RECTANGLE) {
+    hipLaunchKernelGGL(k_lpf_rect, grid, block, 0, st, n, maxi, out);
+  } else {
+    // frequency_filters.jl:69
+    const double sigma = maxi / (static_cast<double>(n) * dx) / pow(2.0 * log(2.0), 1.0 / 6.0);
+    hipLaunchKernelGGL(k_lpf_super_gaussian, grid, block, 0, st, n, sigma, dx, out);
+  }
+  return check_launch();
+}
+
+int holo_apply_low_pass_filter(holo_handle h, void* stream, int nx, int ny, void* out,
+                               const void* in, const float* lpf) {
+  if (!h || !in || !out || !lpf) return fail(HOLO_ERR_ARG, "NULL argument");
+  Geom g;
+  int Ex, Ey;
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  const size_t plane = static_cast<size_t>(nx) * ny;
+  HOLO_TRY(ensure_ws(h, plane * sizeof(float2)));
+  float2* B = static_cast<float2*>(h->ws);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
+  HOLO_TRY(launch_rows_fft(g, Ex, -1, 0, in, 0, B, 0, 1.0f, 1, st));
+  HOLO_TRY(launch_prop_cols(g, Ey, nullptr, B, B, st, lpf));
   return launch_rows_fft(g, Ex, +1, 0, B, 0, static_cast<float2*>(out), 0, sc, 1, st);
 }
 

@@ -124,6 +124,39 @@ __global__ void k_gabor(size_t n, const float* __restrict__ holo, float2* __rest
 }
 
 // ------------------------------------------------------------------------------------------
+// Low-pass filters (frequency_filters.jl), square n x n, centred layout. Element (i, j) of
+// the row-major array (i fast) is the reference's arr[y, x] with y = i + 1, x = j + 1. All
+// arithmetic is FP64 in the reference's order, without contraction; the result is rounded
+// to Float32 on store.
+// ------------------------------------------------------------------------------------------
+// Rectangle window (frequency_filters.jl:5-15, 31-38): 0 where
+// (x - n/2 + 1)^2 + (y - n/2)^2 >= maxi^2, else 1. The two axes are centred one sample
+// apart, as in the reference.
+__global__ void k_lpf_rect(int n, double maxi, float* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || j >= n) return;
+  const double hx = static_cast<double>(j + 1) - 0.5 * n + 1.0;
+  const double hy = static_cast<double>(i + 1) - 0.5 * n;
+  out[static_cast<size_t>(j) * n + i] = (hx * hx + hy * hy >= maxi * maxi) ? 0.f : 1.f;
+}
+
+// Super-Gaussian window of order 3 (frequency_filters.jl:40-50, 66-74):
+// exp(-1/2 * (ax^2 + ay^2)^3) with a = (index - n/2 + 1) / n / dx / sigma on both axes.
+__global__ void k_lpf_super_gaussian(int n, double sigma, double dx, float* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || j >= n) return;
+  const double nd = static_cast<double>(n);
+  const double ax = (static_cast<double>(j + 1) - 0.5 * n + 1.0) / nd / dx / sigma;
+  const double ay = (static_cast<double>(i + 1) - 0.5 * n + 1.0) / nd / dx / sigma;
+  const double s = ax * ax + ay * ay;
+  out[static_cast<size_t>(j) * n + i] = static_cast<float>(exp(-0.5 * (s * s * s)));
+}
+
+// ------------------------------------------------------------------------------------------
 // helpers shared by the pass kernels
 // ------------------------------------------------------------------------------------------
 // LDS of a row pass: `rows` padded lines, then the x plan's twiddle table.
@@ -526,12 +559,15 @@ struct GsTable {
 
 // HMODE 2: natural-order ComplexF32 array from the GS table (precomputed once per GS call),
 // multiplied as is or conjugated (H(-z) = conj(H(z)) for propagating components).
+// HMODE 3: centred real Float32 filter (low-pass filter, frequency_filters.jl:89), each
+// component of the spectrum scaled by it.
 struct PropArgs {
   const float2* in;
   float2* out;
   GenParams gp;        // HMODE 0: zl0 = z / wavlen
   const float2* H;     // HMODE 1: centred array; HMODE 2: natural-order array
   int conj;            // HMODE 2: multiply by conj(H)
+  const float* lpf;    // HMODE 3: centred real filter
 };
 
 template <int E, int L, int HMODE>
@@ -561,12 +597,18 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
   } else if constexpr (HMODE == 1) {
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = cmul(v[e], a.H[centred_index(ci, c.t + T * e, g.nx, g.ny)]);
-  } else {
+  } else if constexpr (HMODE == 2) {
     const float2* __restrict__ H = pd.H;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float2 h = H[static_cast<size_t>(c.t + T * e) * g.nx + ci];
       v[e] = pd.conj ? cmulconj(v[e], h) : cmul(v[e], h);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float l = a.lpf[centred_index(ci, c.t + T * e, g.nx, g.ny)];
+      v[e] = make_float2(l * v[e].x, l * v[e].y);
     }
   }
   fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);

@@ -8,8 +8,10 @@ from . import _lib
 from .holofunc import *  # noqa: F401,F403
 from .holofunc import __all__ as _holofunc_all
 from .holofunc import reconst_xyprojection_holo  # noqa: F401
+from .frequency_filters import *  # noqa: F401,F403
+from .frequency_filters import __all__ as _filters_all
 
-__all__ = list(_holofunc_all) + ["reconst_xyprojection_holo"]
+__all__ = list(_holofunc_all) + list(_filters_all) + ["reconst_xyprojection_holo"]
 
 
 def native_library_path() -> str:

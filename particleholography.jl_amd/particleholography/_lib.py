@@ -15,6 +15,7 @@ LIB_PATH = Path(os.environ.get("HOLO_LIB", PKG_ROOT / "lib" / "libholo.so"))
 HOLO_OK, HOLO_ERR_ARG, HOLO_ERR_SIZE, HOLO_ERR_TYPE, HOLO_ERR_HIP, HOLO_ERR_NOMEM, HOLO_ERR_BUDGET = range(7)
 HOLO_N0F8, HOLO_F16, HOLO_F32, HOLO_F64, HOLO_C64 = range(5)
 HOLO_TF_REFERENCE, HOLO_TF_FP64_D, HOLO_TF_FP64 = range(3)
+HOLO_LPF_RECTANGLE, HOLO_LPF_SUPER_GAUSSIAN = range(2)
 
 
 class ArgumentError(ValueError):
@@ -64,6 +65,8 @@ SIGNATURES = {
     "holo_phase_retrieval": [_vp, _vp, _i, _i, _vp, _vp, _tfp, _tfp, _i, _vp, _i],
     "holo_asm_prop": [_vp, _vp, _i, _i, _vp, _vp, _tfp],
     "holo_fft2": [_vp, _vp, _i, _i, _vp, _vp, _i],
+    "holo_lowpass_filter": [_vp, _vp, _i, _d, _d, _i, _d, _vp],
+    "holo_apply_low_pass_filter": [_vp, _vp, _i, _i, _vp, _vp, _vp],
     "holo_reconst_batch": [_vp, _vp, ctypes.POINTER(holo_job), _i],
     "holo_phase_retrieval_batch": [_vp, _vp, ctypes.POINTER(holo_gs_job), _i],
     "holo_set_tuning_for_size": [_vp, _i, _i, _i, _i, _i, _i, _i],

@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import ArgumentError, check, holo_tf
 
 __all__ = [
-    "CuTransferSqrtPart", "CuTransfer", "CuWavefront", "N0f8", "Float16", "Float32", "Float64",
+    "CuTransferSqrtPart", "CuTransfer", "CuWavefront", "CuLowPassFilter", "N0f8", "Float16", "Float32", "Float64",
     "cu_transfer_sqrt_arr", "cu_transfer", "cu_gabor_wavefront", "cu_phase_retrieval_holo",
     "cu_get_reconst_vol", "cu_get_reconst_xyprojection", "cu_get_reconst_vol_and_xyprojection",
     "cu_get_reconst_complex_vol", "cu_asm_prop_", "cu_asm_prop", "cu_2d_pad",
@@ -85,6 +85,11 @@ class CuTransfer(_Wrapped):
 @dataclass
 class CuWavefront(_Wrapped):
     """Complex wavefront (types.jl:37-39)."""
+
+
+@dataclass
+class CuLowPassFilter(_Wrapped):
+    """Centred real low-pass filter (types.jl:50-52), multiplied with the centred spectrum."""
 
 
 # ---------------------------------------------------------------------------------------

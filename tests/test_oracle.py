@@ -251,3 +251,107 @@ def test_weighted_phase_rms_metric():
     psi = np.exp(1j * np.array([[0.1, -0.2]]))
     ref = np.ones((1, 2), complex)
     assert abs(asm.weighted_phase_rms(psi, ref) - math.sqrt((0.01 + 0.04) / 2)) < 1e-12
+
+
+# ---------------------------------------------------------------------------------------
+# Low-pass filters (frequency_filters.jl)
+# ---------------------------------------------------------------------------------------
+LPF_CASES = [(0.01, 5.3e-7, 128, 6.5e-6),            # runtests.jl:516-535
+             (80000.0 + 100.0 * 1000, LAM, 1024, DX),  # lowpassfilter.md:37
+             (2000.0, LAM, 1024, DX),                  # runtests.jl:424 (pr_dist)
+             (12.0, 1.0, 8, 1.0)]
+
+
+@pytest.mark.parametrize("z,lam,n,dx", LPF_CASES)
+def test_lpf_cutoff_is_the_aperture_angle_frequency(z, lam, n, dx):
+    """Fugal 2009: the highest frequency a point at distance z spreads onto a sensor of
+    width n dx is sin(theta)/wavlen with tan(theta) = (n dx / 2) / z; in samples of the
+    n-point spectrum (spacing 1/(n dx)) that is n dx sin(theta) / wavlen."""
+    theta = math.atan((n * dx / 2) / z)
+    want = n * dx * math.sin(theta) / lam
+    assert abs(asm.lpf_cutoff(z, lam, n, dx) / want - 1) < 1e-12
+
+
+def test_rectangle_filter_hand_mask():
+    """8x8, cutoff radius 2.5 samples: the pass band is the 21 lattice points with
+    dr^2 + dc^2 <= 6 around numpy [2, 3] (Julia x = n/2 - 1 on dim 2, y = n/2 on dim 1)."""
+    n, lam, dx = 8, 1.0, 1.0
+    z = math.sqrt((64.0 / 2.5) ** 2 - 64.0) / 2.0  # cutoff 64 / sqrt(4 z^2 + 64) = 2.5
+    assert abs(asm.lpf_cutoff(z, lam, n, dx) - 2.5) < 1e-12
+    want = np.zeros((n, n), np.float32)
+    for dr in range(-3, 4):
+        for dc in range(-3, 4):
+            if dr * dr + dc * dc <= 6:
+                want[2 + dr, 3 + dc] = 1.0
+    assert want.sum() == 21
+    got = asm.rectangle_filter(z, lam, n, dx)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_rectangle_filter_area():
+    z, lam, n, dx = LPF_CASES[1]
+    maxi = asm.lpf_cutoff(z, lam, n, dx)
+    area = asm.rectangle_filter(z, lam, n, dx).sum()
+    assert abs(area - math.pi * maxi * maxi) < 2 * math.pi * maxi  # lattice count vs disk
+
+
+@pytest.mark.parametrize("z,lam,n,dx", LPF_CASES)
+def test_super_gaussian_half_value_at_cutoff(z, lam, n, dx):
+    maxi = asm.lpf_cutoff(z, lam, n, dx)
+    sig = asm.super_gaussian_sigma(z, lam, n, dx)
+    c = n / 2 - 1  # 1-based centre on both axes
+    assert asm.super_gaussian_at(c, c, n, dx, sig) == 1.0
+    for x, y in ((c + maxi, c), (c, c - maxi), (c + maxi / math.sqrt(2), c + maxi / math.sqrt(2))):
+        assert abs(asm.super_gaussian_at(x, y, n, dx, sig) - 0.5) < 1e-12
+
+
+def test_super_gaussian_grid_symmetry_and_monotonicity():
+    z, lam, n, dx = LPF_CASES[2]
+    f = asm.super_gaussian_filter(z, lam, n, dx)
+    np.testing.assert_array_equal(f, f.T)
+    c = n // 2 - 2  # numpy index of the centre
+    assert f[c, c] == 1.0
+    row = f[c, c:]
+    assert np.all(np.diff(row.astype(np.float64)) <= 0)
+
+
+def _dft2(x, sign):
+    ny, nx = x.shape
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    out = np.zeros((ny, nx), np.complex128)
+    for ky in range(ny):
+        for kx in range(nx):
+            out[ky, kx] = np.sum(x * np.exp(sign * 2j * np.pi * (kx * ii / nx + ky * jj / ny)))
+    return out
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (6, 10), (5, 3)])
+def test_apply_low_pass_filter_bruteforce(shape):
+    """Natural-order spectrum sample (ky, kx) meets filter element
+    ((ky + ny//2) % ny, (kx + nx//2) % nx): the fftshift / ifftshift pair, odd sizes included."""
+    ny, nx = shape
+    rng = np.random.default_rng(ny * 100 + nx)
+    psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    lpf = rng.random(shape)
+    spec = _dft2(psi, -1)
+    for ky in range(ny):
+        for kx in range(nx):
+            spec[ky, kx] *= lpf[(ky + ny // 2) % ny, (kx + nx // 2) % nx]
+    want = _dft2(spec, +1) / (nx * ny)
+    np.testing.assert_allclose(asm.apply_low_pass_filter(psi, lpf), want, atol=1e-12)
+
+
+def test_apply_low_pass_filter_closed_forms():
+    ny, nx = 16, 12
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+    np.testing.assert_allclose(asm.apply_low_pass_filter(psi, np.ones((ny, nx))), psi, atol=1e-13)
+    dc = np.zeros((ny, nx))
+    dc[ny // 2, nx // 2] = 1.0
+    np.testing.assert_allclose(asm.apply_low_pass_filter(psi, dc), np.full((ny, nx), psi.mean()), atol=1e-13)
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    kx, ky = 3, -5
+    wave = np.exp(2j * np.pi * (kx * ii / nx + ky * jj / ny))
+    lpf = rng.random((ny, nx))
+    gain = lpf[(ky % ny + ny // 2) % ny, (kx + nx // 2) % nx]
+    np.testing.assert_allclose(asm.apply_low_pass_filter(wave, lpf), gain * wave, atol=1e-13)
