@@ -164,6 +164,7 @@ def main():
         extras["speedup_vs_plain"] = round(slices_per_s / plain, 2)
         extras["gs_1024"] = bench_gs(ph, synth, torch, dev, d_sqr)
         extras["lpf_1024"] = bench_lpf(ph, torch, holo)
+        extras["ingest_1024"] = bench_ingest(ph, torch, dev)
         extras["batched_64"] = bench_batched(torch, dev, args.batch_holograms)
         extras["serving"] = bench_serving(torch, args.serve_requests)
 
@@ -217,6 +218,28 @@ def bench_lpf(ph, torch, holo):
     _, sec = timed_steps(lambda: ph.cu_apply_low_pass_filter_(wf, lpf), reps, 10, torch)
     return {"apply_us": round(sec / reps * 1e6, 2), "algorithmic_GBs": round(52.0 * N * N * reps / sec / 1e9, 1),
             "config": "1024^2 ComplexF32 wavefront, order-3 super-Gaussian cutoff for 2.1e5 um"}
+
+
+def bench_ingest(ph, torch, dev):
+    """8-bit ingest at 1024^2: 64 device-resident uint8 holograms converted to Float32 with the
+    background removal fused (1 B in + 8 B background + 4 B out per pixel), and the per-pixel
+    background mode / mean of a 256-image stack (1 B per pixel per image, read once)."""
+    g = torch.Generator(device="cpu").manual_seed(17)
+    stack = torch.randint(0, 256, (256, N, N), dtype=torch.uint8, generator=g).to(dev)
+    bkg = torch.rand((N, N), dtype=torch.float64, generator=g).to(dev)
+    batch = stack[:64]
+    reps = 20
+    _, t_ing = timed_steps(lambda: ph.cu_ingest_u8(batch, bkg=bkg), reps, 2, torch)
+    _, t_mode = timed_steps(lambda: ph.cu_make_background(stack, "mode"), reps, 2, torch)
+    _, t_mean = timed_steps(lambda: ph.cu_make_background(stack, "mean"), reps, 2, torch)
+    px = float(N * N)
+    return {"ingest_holograms_per_s": round(64 * reps / t_ing, 1),
+            "ingest_GBs": round(13.0 * px * 64 * reps / t_ing / 1e9, 1),
+            "background_mode_ms": round(t_mode / reps * 1e3, 3),
+            "background_mode_GBs": round(px * 256 * reps / t_mode / 1e9, 1),
+            "background_mean_ms": round(t_mean / reps * 1e3, 3),
+            "background_mean_GBs": round(px * 256 * reps / t_mean / 1e9, 1),
+            "config": "1024^2 uint8 holograms: 64 ingested with a Float64 background, 256 reduced"}
 
 
 def bench_batched(torch, dev, nholo):

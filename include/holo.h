@@ -1,6 +1,7 @@
 /* holo.h -- C ABI of the MI355X angular-spectrum / Gerchberg-Saxton library (libholo.so).
  *
- * Every entry point replaces one function of ParticleHolography.jl's src/holofunc.jl; the
+ * Every entry point replaces one function of ParticleHolography.jl (src/holofunc.jl,
+ * src/frequency_filters.jl, src/utils.jl) or says that it has no counterpart there; the
  * comment above each declaration names it (file:line). All pointers are device pointers
  * (HIP global memory) unless stated; the caller allocates every input and output. Calls are
  * asynchronous and ordered on the given `stream` (a hipStream_t, NULL = the null stream).
@@ -162,6 +163,25 @@ int holo_lowpass_filter(holo_handle h, void* stream, int kind, double prop_dist,
  * out may alias in (cu_apply_low_pass_filter! passes the same buffer). */
 int holo_apply_low_pass_filter(holo_handle h, void* stream, int nx, int ny, void* out,
                                const void* in, const float* lpf);
+
+/* ---- ingest: 8-bit holograms to Float32 (load_gray2float, utils.jl:23-25) with the tutorial's
+ *      background removal img .- bkg .+ 64/255 (preprocessings.md:21-23) ------------------- */
+/* `count` uint8 images of nx*ny pixels (image m at src + m*nx*ny, raw N0f8 bytes) to Float32:
+ * out = Float32((Float64(Float32(v/255)) - bkg) + offset); bkg (Float64 nx*ny, shared by all
+ * images) may be NULL, meaning 0, and then offset 0 gives exactly load_gray2float's v/255. */
+int holo_ingest_u8(holo_handle h, void* stream, int nx, int ny, int count, const uint8_t* src,
+                   const double* bkg, double offset, float* out);
+
+/* ---- background of an image stack: cu_make_background_mode (utils.jl:243-317) and
+ *      make_background(...; mode=:mean) (utils.jl:217-224) --------------------------------- */
+enum holo_background_kind {
+  HOLO_BACKGROUND_MODE = 0,  /* per-pixel most frequent value (smallest on ties) / 255.0 */
+  HOLO_BACKGROUND_MEAN = 1   /* per-pixel FP64 sum of v/255 in image order, / count */
+};
+/* out (Float64 nx*ny) from `count` >= 1 uint8 images (image m at stack + m*nx*ny); the mode
+ * takes at most 65535 images per call. */
+int holo_make_background(holo_handle h, void* stream, int nx, int ny, int count,
+                         const uint8_t* stack, int kind, double* out);
 
 /* ---- batched reconstruction (no counterpart in the reference: its functions take one
  *      hologram; this runs many cu_get_reconst_xyprojection calls, holofunc.jl:248-266, together) */

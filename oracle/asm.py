@@ -367,3 +367,56 @@ PARITY = {
     "apply_low_pass_filter": "all-ones filter = identity, DC-only filter = mean, single-frequency "
                              "eigenfunction scaled by its filter value, O(N^4) brute-force DFT",
 }
+
+
+# --------------------------------------------------------------------------------------
+# Ingest: load_gray2float, make_background, cu_make_background_mode (utils.jl:23-25,
+# 217-317) and the tutorial's background removal (preprocessings.md:21-23)
+# --------------------------------------------------------------------------------------
+def n0f8_to_f32(u8) -> np.ndarray:
+    """Float32(reinterpret(N0f8, v)) = v / 255 correctly rounded to Float32."""
+    return np.asarray(u8, np.uint8).astype(np.float32) / np.float32(255)
+
+
+def ingest_u8(u8, bkg=None, offset: float = 0.0) -> np.ndarray:
+    """Float32((Float64(v/255) - bkg) + offset) with bkg 0 when absent; with a background and
+    offset 64/255 this is ``img .= img .- bkg .+ 64/255`` on a Float32 img."""
+    f = n0f8_to_f32(u8).astype(np.float64)
+    b = 0.0 if bkg is None else np.asarray(bkg, np.float64)
+    return ((f - b) + offset).astype(np.float32)
+
+
+def background_mode(stack) -> np.ndarray:
+    """Per-pixel mode of a uint8 stack (count, ...) / 255.0: votes per value, then the first
+    (smallest) value with the largest count (utils.jl:229-237, 254-271, 313-315)."""
+    s = np.asarray(stack, np.uint8)
+    flat = s.reshape(s.shape[0], -1)
+    npx = flat.shape[1]
+    mode = np.empty(npx, np.int64)
+    chunk = 1 << 16
+    for c0 in range(0, npx, chunk):
+        part = flat[:, c0:c0 + chunk].astype(np.int64)
+        w = part.shape[1]
+        idx = part * w + np.arange(w)[None, :]  # vote (value, pixel) -> value * w + pixel
+        votes = np.bincount(idx.ravel(), minlength=256 * w).reshape(256, w)
+        mode[c0:c0 + w] = np.argmax(votes, axis=0)  # first maximum = smallest value
+    return (mode.astype(np.float64) / 255.0).reshape(s.shape[1:])
+
+
+def background_mean(stack) -> np.ndarray:
+    """Per-pixel FP64 sum of Float32(v/255) in image order from 0.0, then / count
+    (utils.jl:219-224)."""
+    s = np.asarray(stack, np.uint8)
+    acc = np.zeros(s.shape[1:], np.float64)
+    for img in s:
+        acc = acc + n0f8_to_f32(img).astype(np.float64)
+    return acc / s.shape[0]
+
+
+PARITY.update({
+    "n0f8_to_f32": "exhaustive over 0..255 against the exact rational v/255 rounded to Float32",
+    "ingest_u8": "scalar re-evaluation with Python floats (IEEE FP64) on every byte value",
+    "background_mode": "cu_make_background_mode known-answer test (runtests.jl:125-138), "
+                       "collections.Counter brute force with smallest-value tie break",
+    "background_mean": "sequential Python-float sum, and math.fsum mean within 1e-15 relative",
+})

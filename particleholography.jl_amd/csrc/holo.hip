@@ -19,6 +19,7 @@
 
 #include "holo.h"
 #include "holo_kernels.hpp"
+#include "ingest_kernels.hpp"
 
 using namespace holo;
 
@@ -921,6 +922,48 @@ int holo_apply_low_pass_filter(holo_handle h, void* stream, int nx, int ny, void
   HOLO_TRY(launch_rows_fft(g, Ex, -1, 0, in, 0, B, 0, 1.0f, 1, st));
   HOLO_TRY(launch_prop_cols(g, Ey, nullptr, B, B, st, lpf));
   return launch_rows_fft(g, Ex, +1, 0, B, 0, static_cast<float2*>(out), 0, sc, 1, st);
+}
+
+int holo_ingest_u8(holo_handle h, void* stream, int nx, int ny, int count, const uint8_t* src,
+                   const double* bkg, double offset, float* out) {
+  if (!h || !src || !out) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (nx < 1 || ny < 1 || count < 0) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const unsigned gx = static_cast<unsigned>((npx + 4 * 256 - 1) / (4 * 256));
+  for (int m0 = 0; m0 < count; m0 += 65535) {
+    const int c = std::min(65535, count - m0);
+    const uint8_t* s = src + static_cast<size_t>(m0) * npx;
+    float* o = out + static_cast<size_t>(m0) * npx;
+    const int vec = (npx % 4 == 0) && (reinterpret_cast<uintptr_t>(s) % 4 == 0) &&
+                    (reinterpret_cast<uintptr_t>(o) % 16 == 0);
+    hipLaunchKernelGGL(k_ingest_u8, dim3(gx, c), dim3(256), 0, st, npx, c, s, bkg, offset, o, vec);
+    HOLO_TRY(check_launch());
+  }
+  return HOLO_OK;
+}
+
+int holo_make_background(holo_handle h, void* stream, int nx, int ny, int count,
+                         const uint8_t* stack, int kind, double* out) {
+  if (!h || !stack || !out) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  if (count < 1) return fail(HOLO_ERR_ARG, "the image list is empty");
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (kind == HOLO_BACKGROUND_MODE) {
+    if (count > 65535) return fail(HOLO_ERR_SIZE, "at most 65535 images per background-mode call");
+    const int vec = (npx % 2 == 0) && (reinterpret_cast<uintptr_t>(stack) % 2 == 0);
+    const unsigned gx = static_cast<unsigned>((npx + 2 * kModePairs - 1) / (2 * kModePairs));
+    hipLaunchKernelGGL(k_background_mode, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out, vec);
+  } else if (kind == HOLO_BACKGROUND_MEAN) {
+    const int vec = (npx % 4 == 0) && (reinterpret_cast<uintptr_t>(stack) % 4 == 0);
+    const size_t threads = vec ? npx / 4 : npx;
+    hipLaunchKernelGGL(k_background_mean, dim3(static_cast<unsigned>((threads + 255) / 256)), dim3(256), 0, st,
+                       npx, count, stack, out, vec);
+  } else {
+    return fail(HOLO_ERR_ARG, "kind must be HOLO_BACKGROUND_MODE or HOLO_BACKGROUND_MEAN");
+  }
+  return check_launch();
 }
 
 int holo_reconst_xyprojection(holo_handle h, void* stream, int nx, int ny, const void* wavefront,

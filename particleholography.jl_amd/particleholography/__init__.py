@@ -10,8 +10,10 @@ from .holofunc import __all__ as _holofunc_all
 from .holofunc import reconst_xyprojection_holo  # noqa: F401
 from .frequency_filters import *  # noqa: F401,F403
 from .frequency_filters import __all__ as _filters_all
+from .utils import *  # noqa: F401,F403
+from .utils import __all__ as _utils_all
 
-__all__ = list(_holofunc_all) + list(_filters_all) + ["reconst_xyprojection_holo"]
+__all__ = list(_holofunc_all) + list(_filters_all) + list(_utils_all) + ["reconst_xyprojection_holo"]
 
 
 def native_library_path() -> str:

@@ -16,6 +16,7 @@ HOLO_OK, HOLO_ERR_ARG, HOLO_ERR_SIZE, HOLO_ERR_TYPE, HOLO_ERR_HIP, HOLO_ERR_NOME
 HOLO_N0F8, HOLO_F16, HOLO_F32, HOLO_F64, HOLO_C64 = range(5)
 HOLO_TF_REFERENCE, HOLO_TF_FP64_D, HOLO_TF_FP64 = range(3)
 HOLO_LPF_RECTANGLE, HOLO_LPF_SUPER_GAUSSIAN = range(2)
+HOLO_BACKGROUND_MODE, HOLO_BACKGROUND_MEAN = range(2)
 
 
 class ArgumentError(ValueError):
@@ -67,6 +68,8 @@ SIGNATURES = {
     "holo_fft2": [_vp, _vp, _i, _i, _vp, _vp, _i],
     "holo_lowpass_filter": [_vp, _vp, _i, _d, _d, _i, _d, _vp],
     "holo_apply_low_pass_filter": [_vp, _vp, _i, _i, _vp, _vp, _vp],
+    "holo_ingest_u8": [_vp, _vp, _i, _i, _i, _vp, _vp, _d, _vp],
+    "holo_make_background": [_vp, _vp, _i, _i, _i, _vp, _i, _vp],
     "holo_reconst_batch": [_vp, _vp, ctypes.POINTER(holo_job), _i],
     "holo_phase_retrieval_batch": [_vp, _vp, ctypes.POINTER(holo_gs_job), _i],
     "holo_set_tuning_for_size": [_vp, _i, _i, _i, _i, _i, _i, _i],

@@ -355,3 +355,80 @@ def test_apply_low_pass_filter_closed_forms():
     lpf = rng.random((ny, nx))
     gain = lpf[(ky % ny + ny // 2) % ny, (kx + nx // 2) % nx]
     np.testing.assert_allclose(asm.apply_low_pass_filter(wave, lpf), gain * wave, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------------------
+# Ingest (utils.jl:23-25, 217-317; preprocessings.md:21-23)
+# ---------------------------------------------------------------------------------------
+def test_n0f8_to_f32_exhaustive():
+    from fractions import Fraction
+
+    got = asm.n0f8_to_f32(np.arange(256))
+    for v in range(256):
+        exact = Fraction(v, 255)
+        f = float(got[v])
+        # correctly rounded: no other float32 is closer to v/255
+        lo = float(np.nextafter(np.float32(f), np.float32(-1)))
+        hi = float(np.nextafter(np.float32(f), np.float32(2)))
+        d = abs(Fraction(f) - exact)
+        assert d <= abs(Fraction(lo) - exact) and d <= abs(Fraction(hi) - exact), v
+
+
+def test_ingest_scalar_reevaluation():
+    rng = np.random.default_rng(3)
+    u8 = np.arange(256, dtype=np.uint8).reshape(16, 16)
+    bkg = rng.random((16, 16))
+    off = 64 / 255
+    got = asm.ingest_u8(u8, bkg, off)
+    for j in range(16):
+        for i in range(16):
+            f = float(np.float32(int(u8[j, i]) / 255))  # v/255 has no rounding tie at 24 bits
+            want = np.float32((f - float(bkg[j, i])) + off)
+            assert got[j, i] == want
+    np.testing.assert_array_equal(asm.ingest_u8(u8), asm.n0f8_to_f32(u8))
+
+
+def test_background_mode_known_answer():
+    """runtests.jl:125-138, Julia matrices [10 20; 30 40], [10 25; 30 45], [10 20; 30 40] in
+    the reference layout (tensor [j, i] = Julia [i+1, j+1])."""
+    imgs = [np.array([[10, 30], [20, 40]], np.uint8), np.array([[10, 30], [25, 45]], np.uint8),
+            np.array([[10, 30], [20, 40]], np.uint8)]
+    bg = asm.background_mode(np.stack(imgs))
+    assert bg.dtype == np.float64 and bg.shape == (2, 2)
+    for (j, i), v in {(0, 0): 10, (1, 0): 20, (0, 1): 30, (1, 1): 40}.items():
+        assert abs(bg[j, i] - v / 255) < 1e-6
+
+
+def test_background_mode_counter_bruteforce():
+    from collections import Counter
+
+    rng = np.random.default_rng(11)
+    stack = rng.integers(0, 4, size=(9, 7, 5)).astype(np.uint8) * 60  # frequent ties
+    bg = asm.background_mode(stack)
+    for j in range(7):
+        for i in range(5):
+            c = Counter(int(v) for v in stack[:, j, i])
+            top = max(c.values())
+            assert bg[j, i] == min(v for v, n in c.items() if n == top) / 255.0
+
+
+def test_background_mean_sequential_and_fsum():
+    rng = np.random.default_rng(12)
+    stack = rng.integers(0, 256, size=(13, 4, 6)).astype(np.uint8)
+    bg = asm.background_mean(stack)
+    for j in range(4):
+        for i in range(6):
+            acc = 0.0
+            for m in range(13):
+                acc = acc + float(np.float32(int(stack[m, j, i]) / 255))
+            assert bg[j, i] == acc / 13
+            exact = math.fsum(float(np.float32(int(stack[m, j, i]) / 255)) for m in range(13)) / 13
+            assert abs(bg[j, i] - exact) <= 1e-15 * max(exact, 1e-300)
+
+
+def test_n0f8_reciprocal_product_rounds_like_the_quotient():
+    """The device conversion rounds the FP64 product v * fl(1/255) to Float32; for every byte
+    this equals the correctly rounded quotient the oracle uses."""
+    v = np.arange(256)
+    prod = (v.astype(np.float64) * (1.0 / 255.0)).astype(np.float32)
+    np.testing.assert_array_equal(prod, asm.n0f8_to_f32(v))
