@@ -8,7 +8,9 @@ module ParticleHolographyHIP
 using AMDGPU
 using FixedPointNumbers
 
-export CuTransferSqrtPart, CuTransfer, CuWavefront
+export CuTransferSqrtPart, CuTransfer, CuWavefront, CuLowPassFilter
+export cu_rectangle_filter, cu_super_gaussian_filter, cu_apply_low_pass_filter, cu_apply_low_pass_filter!
+export cu_make_background_mode, cu_ingest_u8
 export cu_transfer_sqrt_arr, cu_transfer, cu_gabor_wavefront, cu_phase_retrieval_holo,
        cu_get_reconst_vol, cu_get_reconst_xyprojection, cu_get_reconst_vol_and_xyprojection,
        cu_get_reconst_complex_vol, cu_asm_prop!, cu_2d_pad, cu_get_reconst_vol_and_xyprojection_padded
@@ -35,7 +37,11 @@ struct CuWavefront{T<:Complex}
     data::ROCArray{T,2}
 end
 
-for T in (CuTransferSqrtPart, CuTransfer, CuWavefront)
+struct CuLowPassFilter{T<:AbstractFloat}
+    data::ROCArray{T,2}
+end
+
+for T in (CuTransferSqrtPart, CuTransfer, CuWavefront, CuLowPassFilter)
     @eval begin
         Base.size(x::$T) = size(x.data)
         Base.axes(x::$T) = axes(x.data)
@@ -200,6 +206,68 @@ function cu_asm_prop!(outholo::CuWavefront, inholo::CuWavefront, d_sqr::CuTransf
                  (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ref{HoloTF}),
                  _handle(), _stream(), datlen, datlen, pointer(outholo.data), pointer(inholo.data), tf))
     return nothing
+end
+
+# ---- frequency_filters.jl:31-107 -------------------------------------------------------------
+const HOLO_LPF_RECTANGLE, HOLO_LPF_SUPER_GAUSSIAN = 0, 1
+
+function _lpf(kind, prop_dist, wavlen, imglen, pixel_picth)
+    arr = ROCArray{Float32}(undef, imglen, imglen)
+    _check(ccall((:holo_lowpass_filter, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Float64, Float64, Cint, Float64, Ptr{Cvoid}),
+                 _handle(), _stream(), kind, prop_dist, wavlen, imglen, pixel_picth, pointer(arr)))
+    return CuLowPassFilter(arr)
+end
+cu_rectangle_filter(prop_dist::AbstractFloat, wavlen::AbstractFloat, imglen::Int, pixel_picth::AbstractFloat) =
+    _lpf(HOLO_LPF_RECTANGLE, prop_dist, wavlen, imglen, pixel_picth)
+cu_super_gaussian_filter(prop_dist::AbstractFloat, wavlen::AbstractFloat, imglen::Int, pixel_picth::AbstractFloat) =
+    _lpf(HOLO_LPF_SUPER_GAUSSIAN, prop_dist, wavlen, imglen, pixel_picth)
+
+function _apply_lpf!(out::ROCArray, holo::CuWavefront, lpf::CuLowPassFilter{Float32})
+    @assert size(holo.data) == size(lpf.data) "low-pass filter and wavefront must have the same size"
+    nx, ny = size(holo.data)
+    _check(ccall((:holo_apply_low_pass_filter, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                 _handle(), _stream(), nx, ny, pointer(out), pointer(holo.data), pointer(lpf.data)))
+    return out
+end
+function cu_apply_low_pass_filter!(holo::CuWavefront, lpf::CuLowPassFilter)
+    _apply_lpf!(holo.data, holo, lpf)
+    return nothing
+end
+cu_apply_low_pass_filter(holo::CuWavefront, lpf::CuLowPassFilter) =
+    CuWavefront(_apply_lpf!(similar(holo.data), holo, lpf))
+
+# ---- utils.jl:286-317 (cu_make_background_mode) and 8-bit ingest ------------------------------
+const HOLO_BACKGROUND_MODE, HOLO_BACKGROUND_MEAN = 0, 1
+
+function cu_make_background_mode(grayimglist)
+    H, W = size(grayimglist[1])
+    stack = Array{UInt8}(undef, H, W, length(grayimglist))
+    for (k, img) in enumerate(grayimglist)
+        stack[:, :, k] .= eltype(img) === UInt8 ? img : round.(UInt8, clamp.(Float32.(img) .* 255, 0, 255))
+    end
+    d_stack = ROCArray(stack)
+    out = ROCArray{Float64}(undef, H, W)
+    _check(ccall((:holo_make_background, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Cint, Ptr{Cvoid}),
+                 _handle(), _stream(), H, W, length(grayimglist), pointer(d_stack), HOLO_BACKGROUND_MODE,
+                 pointer(out)))
+    return Array(out)
+end
+
+# Float32 holograms from raw 8-bit images on the device: v/255, or Float32((v/255 - bkg) + offset)
+# with a Float64 background (preprocessings.md:21-23). No counterpart function in the reference.
+function cu_ingest_u8(imgs::ROCArray{UInt8}, bkg::Union{Nothing,ROCArray{Float64,2}}=nothing,
+                      offset::Float64=(bkg === nothing ? 0.0 : 64 / 255))
+    nx, ny = size(imgs, 1), size(imgs, 2)
+    count = ndims(imgs) == 2 ? 1 : size(imgs, 3)
+    out = ROCArray{Float32}(undef, size(imgs))
+    _check(ccall((:holo_ingest_u8, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
+                 _handle(), _stream(), nx, ny, count, pointer(imgs),
+                 bkg === nothing ? C_NULL : pointer(bkg), offset, pointer(out)))
+    return out
 end
 
 end # module
