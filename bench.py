@@ -165,6 +165,7 @@ def main():
         extras["gs_1024"] = bench_gs(ph, synth, torch, dev, d_sqr)
         extras["lpf_1024"] = bench_lpf(ph, torch, holo)
         extras["ingest_1024"] = bench_ingest(ph, torch, dev)
+        extras["detect_1024x1000"] = bench_detect(ph, torch, holo)
         extras["batched_64"] = bench_batched(torch, dev, args.batch_holograms)
         extras["serving"] = bench_serving(torch, args.serve_requests)
 
@@ -240,6 +241,41 @@ def bench_ingest(ph, torch, dev):
             "background_mean_ms": round(t_mean / reps * 1e3, 3),
             "background_mean_GBs": round(px * 256 * reps / t_mean / 1e9, 1),
             "config": "1024^2 uint8 holograms: 64 ingested with a Float64 background, 256 reduced"}
+
+
+def bench_detect(ph, torch, holo):
+    """Particle detection downstream of reconstruction: the 1024^2 x 1000 N0f8 volume of the
+    headline hologram, thresholded at 30/255 inside the labelling kernel, 8-connected
+    components and their rectangles for every slice (one libholo call; 1 B read per voxel plus
+    the union-find state), then the host z-linking into neighbourhoods."""
+    import ctypes
+
+    from particleholography import _lib
+
+    d_sqr = ph.cu_transfer_sqrt_arr(N, LAM, DX)
+    h0, hdz = ph.cu_transfer(-Z0, N, LAM, d_sqr), ph.cu_transfer(-DZ, N, LAM, d_sqr)
+    vol = ph.cu_get_reconst_vol(ph.cu_gabor_wavefront(holo), h0, hdz, SLICES)
+    max_boxes = 4096
+    boxes = torch.empty((SLICES, max_boxes, 5), dtype=torch.int32, device=vol.device)
+    nbox = torch.empty(SLICES, dtype=torch.int32, device=vol.device)
+    lib, h = _lib.load(), _lib.handle(0)
+
+    def run():
+        st = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib.holo_ccl_boxes(h.ptr, st, N, N, SLICES, vol.data_ptr(), _lib.HOLO_CCL_N0F8_LE,
+                                      30 / 255, max_boxes, boxes.data_ptr(), nbox.data_ptr()))
+
+    reps = 5
+    _, sec = timed_steps(run, reps, 1, torch)
+    t0 = time.perf_counter()
+    pbbs = ph.particle_bounding_boxes(vol, threshold=30 / 255)
+    total = time.perf_counter() - t0
+    return {"label_boxes_ms": round(sec / reps * 1e3, 3),
+            "slices_per_s": round(SLICES * reps / sec, 1),
+            "voxel_GBs": round(float(N * N) * SLICES * reps / sec / 1e9, 1),
+            "components": int(nbox.sum().item()), "neighbourhoods": len(pbbs),
+            "particle_bounding_boxes_ms": round(total * 1e3, 2),
+            "config": "1024^2 x 1000 N0f8 volume, threshold 30/255, 8-connected, max 4096 boxes per slice"}
 
 
 def bench_batched(torch, dev, nholo):

@@ -183,6 +183,38 @@ enum holo_background_kind {
 int holo_make_background(holo_handle h, void* stream, int nx, int ny, int count,
                          const uint8_t* stack, int kind, double* out);
 
+/* ---- connected components and bounding boxes of slices: cu_connected_component_labeling,
+ *      cu_find_valid_labels, get_bounding_rectangles (ccl.jl:186-264) and the per-slice part of
+ *      particle_bounding_boxes (particle_detection.jl:58-96); cu_dilate (particle_detection.jl:12-42) */
+enum holo_ccl_kind {
+  HOLO_CCL_F32 = 0,     /* Float32 image: values > 0 are foreground; 8-neighbours with equal
+                           values are connected (cu_connected_component_labeling on a Float32 image) */
+  HOLO_CCL_BINARY = 1,  /* uint8 / Bool image: nonzero is foreground (a slice of d_bin_vol) */
+  HOLO_CCL_N0F8_LE = 2  /* uint8 N0f8 volume thresholded on the fly: foreground where
+                           v/255 <= threshold (d_vol .<= threshold) */
+};
+/* One component of one slice: label = 1 + the smallest offset j*nx + i of its pixels (the
+ * reference's label), coordinates 0-based and inclusive (i fast, j slow). */
+typedef struct holo_box {
+  uint32_t label;
+  int32_t i0, j0, i1, j1;
+} holo_box;
+/* labels (uint32, count slices of nx*ny): 1 + the smallest offset in the pixel's 8-connected
+ * component, 0 for background. Images hold at most 2^32-2 pixels. */
+int holo_ccl_label(holo_handle h, void* stream, int nx, int ny, int count, const void* img, int kind,
+                   double threshold, uint32_t* labels);
+/* Boxes of every component of every slice without a label image: slice z's boxes at
+ * boxes + z*max_boxes in no particular order, nboxes[z] their number (device int[count]); a
+ * slice with more than max_boxes components stores the first max_boxes claimed and reports
+ * the full count. */
+int holo_ccl_boxes(holo_handle h, void* stream, int nx, int ny, int count, const void* img, int kind,
+                   double threshold, int max_boxes, holo_box* boxes, int* nboxes);
+/* The same boxes from a label image (get_bounding_rectangles on every valid label). */
+int holo_label_boxes(holo_handle h, void* stream, int nx, int ny, int count, const uint32_t* labels,
+                     int max_boxes, holo_box* boxes, int* nboxes);
+/* out = 3 x 3 in-plane dilation of each slice (uint8 / Bool), false on the slice border. */
+int holo_dilate(holo_handle h, void* stream, int nx, int ny, int count, const uint8_t* in, uint8_t* out);
+
 /* ---- batched reconstruction (no counterpart in the reference: its functions take one
  *      hologram; this runs many cu_get_reconst_xyprojection calls, holofunc.jl:248-266, together) */
 typedef struct holo_job {

@@ -1,0 +1,327 @@
+// Connected-component labelling, bounding boxes and dilation of slices (ccl.jl:186-264,
+// particle_detection.jl:12-96).
+//
+// A slice is an nx x ny plane (x fastest); `count` slices lie back to back. A pixel's class
+// is 0 (background) or a nonzero key; 8-neighbours with equal nonzero keys are connected.
+// The label of a component is 1 + the smallest memory offset among its pixels, background
+// pixels are 0: the labels the reference's union-find produces (every union links the
+// larger root under the smaller, so the surviving root is the first pixel in memory order).
+//
+// Three passes over a slice:
+//   1. k_ccl_local: a workgroup owns a 64 x 16 tile, unions its pixels in LDS and writes for
+//      every foreground pixel the global offset of its tile-local root (P), background
+//      0xffffffff;
+//   2. k_ccl_border: unions across tile edges only (W and NW of the left column, NW/N/NE of
+//      the top row, NE of the right column), lock-free on P with atomicMin;
+//   3. k_ccl_labels (labels = root + 1), or k_ccl_roots + k_ccl_boxes (one box slot per root,
+//      then min / max of the coordinates of every run of equal roots along a row).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace holo {
+
+constexpr int kTileX = 64;
+constexpr int kTileY = 16;
+constexpr int kTilePx = kTileX * kTileY;
+constexpr unsigned kNoLabel = 0xffffffffu;
+
+enum CclKind : int {
+  CCL_F32 = 0,     // Float32 image: positive values are foreground, equal values connect
+  CCL_BINARY = 1,  // uint8 / Bool image: nonzero is foreground
+  CCL_N0F8_LE = 2  // uint8 (N0f8) volume: v < lim is foreground (d_vol .<= threshold: the
+                   // bytes whose v/255 <= threshold are 0 .. lim-1)
+};
+
+struct CclSrc {
+  const void* img;   // slice z at img + z * npx elements
+  int kind;
+  unsigned lim;      // CCL_N0F8_LE: number of foreground byte values
+};
+
+__device__ __forceinline__ unsigned ccl_key(const CclSrc& s, size_t idx) {
+  if (s.kind == CCL_F32) {
+    const float v = static_cast<const float*>(s.img)[idx];
+    return v > 0.0f ? __float_as_uint(v) : 0u;  // positive floats: equal values <=> equal bits
+  }
+  const unsigned v = static_cast<const uint8_t*>(s.img)[idx];
+  if (s.kind == CCL_BINARY) return v != 0u ? 1u : 0u;
+  return v < s.lim ? 1u : 0u;
+}
+
+__device__ __forceinline__ unsigned lds_find(volatile unsigned* par, unsigned x) {
+  unsigned p = par[x];
+  while (p != x) {
+    x = p;
+    p = par[x];
+  }
+  return x;
+}
+
+// Links the roots of a and b, the larger under the smaller (lock-free: a failed atomicMin
+// means a's root changed under us, and the loop continues from its new parent).
+__device__ __forceinline__ void lds_union(unsigned* par, unsigned a, unsigned b) {
+  volatile unsigned* vp = par;
+  while (true) {
+    a = lds_find(vp, a);
+    b = lds_find(vp, b);
+    if (a == b) return;
+    if (a < b) {
+      const unsigned t = a;
+      a = b;
+      b = t;
+    }
+    const unsigned old = atomicMin(&par[a], b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+__device__ __forceinline__ unsigned g_load(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned g_find(unsigned* P, unsigned x) {
+  unsigned p = g_load(P + x);
+  while (p != x) {
+    x = p;
+    p = g_load(P + x);
+  }
+  return x;
+}
+
+__device__ __forceinline__ void g_union(unsigned* P, unsigned a, unsigned b) {
+  while (true) {
+    a = g_find(P, a);
+    b = g_find(P, b);
+    if (a == b) return;
+    if (a < b) {
+      const unsigned t = a;
+      a = b;
+      b = t;
+    }
+    const unsigned old = atomicMin(P + a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+// Pass 1. grid (ceil(nx/64), ceil(ny/16), count), 256 threads: thread t owns column t % 64 of
+// rows t/64 + 4k, k = 0..3.
+__global__ __launch_bounds__(256) void k_ccl_local(int nx, int ny, CclSrc src, unsigned* __restrict__ P) {
+  __shared__ unsigned key[kTilePx];
+  __shared__ unsigned par[kTilePx];
+  const int t = threadIdx.x;
+  const int lx = t % kTileX;
+  const int x = blockIdx.x * kTileX + lx;
+  const int y0 = blockIdx.y * kTileY;
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const size_t base = static_cast<size_t>(blockIdx.z) * npx;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ly = t / kTileX + 4 * k;
+    const int y = y0 + ly;
+    const int i = ly * kTileX + lx;
+    key[i] = (x < nx && y < ny) ? ccl_key(src, base + static_cast<size_t>(y) * nx + x) : 0u;
+    par[i] = i;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ly = t / kTileX + 4 * k;
+    const int i = ly * kTileX + lx;
+    const unsigned c = key[i];
+    if (c == 0u) continue;
+    if (lx > 0 && key[i - 1] == c) lds_union(par, i, i - 1);
+    if (ly > 0) {
+      if (lx > 0 && key[i - kTileX - 1] == c) lds_union(par, i, i - kTileX - 1);
+      if (key[i - kTileX] == c) lds_union(par, i, i - kTileX);
+      if (lx < kTileX - 1 && key[i - kTileX + 1] == c) lds_union(par, i, i - kTileX + 1);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ly = t / kTileX + 4 * k;
+    const int y = y0 + ly;
+    if (x >= nx || y >= ny) continue;
+    const int i = ly * kTileX + lx;
+    unsigned out = kNoLabel;
+    if (key[i] != 0u) {
+      const unsigned r = lds_find(par, i);
+      out = static_cast<unsigned>((y0 + static_cast<int>(r) / kTileX) * nx + blockIdx.x * kTileX + r % kTileX);
+    }
+    P[base + static_cast<size_t>(y) * nx + x] = out;
+  }
+}
+
+// Pass 2. grid (ceil(nx/64), ceil(ny/16), count), 128 threads: t < 64 the top-row pixel
+// lx = t (NW, N, NE across the top edge), 64 <= t < 80 the left-column pixel ly = t - 64
+// (W, NW), 80 <= t < 96 the right-column pixel ly = t - 80 (NE).
+__global__ __launch_bounds__(128) void k_ccl_border(int nx, int ny, CclSrc src, unsigned* __restrict__ P) {
+  const int t = threadIdx.x;
+  const int x0 = blockIdx.x * kTileX, y0 = blockIdx.y * kTileY;
+  int x, y;
+  if (t < 64) {
+    x = x0 + t;
+    y = y0;
+  } else if (t < 80) {
+    x = x0;
+    y = y0 + (t - 64);
+  } else if (t < 96) {
+    x = x0 + kTileX - 1;
+    y = y0 + (t - 80);
+  } else {
+    return;
+  }
+  if (x >= nx || y >= ny) return;
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const size_t base = static_cast<size_t>(blockIdx.z) * npx;
+  unsigned* Ps = P + base;
+  const unsigned me = static_cast<unsigned>(y) * nx + x;
+  const unsigned c = ccl_key(src, base + me);
+  if (c == 0u) return;
+  auto link = [&](int qx, int qy) {
+    if (qx < 0 || qx >= nx || qy < 0) return;
+    const unsigned q = static_cast<unsigned>(qy) * nx + qx;
+    if (ccl_key(src, base + q) == c) g_union(Ps, me, q);
+  };
+  if (t < 64) {
+    if (y > 0) {
+      link(x - 1, y - 1);
+      link(x, y - 1);
+      link(x + 1, y - 1);
+    }
+  } else if (t < 80) {
+    link(x - 1, y);
+    link(x - 1, y - 1);
+  } else {
+    link(x + 1, y - 1);
+  }
+}
+
+// Pass 3 (labels): labels = root + 1 for foreground pixels, 0 for background. One thread
+// per pixel; P is compressed to the root on the way.
+__global__ void k_ccl_labels(size_t npx, int count, unsigned* __restrict__ P, uint32_t* __restrict__ labels) {
+  const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t z = blockIdx.y;
+  if (idx >= npx || static_cast<int>(z) >= count) return;
+  unsigned* Ps = P + z * npx;
+  const unsigned p = Ps[idx];
+  uint32_t out = 0u;
+  if (p != kNoLabel) {
+    const unsigned r = g_find(Ps, p);
+    Ps[idx] = r;
+    out = r + 1u;
+  }
+  labels[z * npx + idx] = out;
+}
+
+// P from a label image. A pixel joins label l only when l is a valid label -- the pixel at
+// offset l - 1 carries l itself (cu_find_valid_labels' test, ccl.jl:240-248) -- so every entry
+// of P is a self-pointing root or kNoLabel, whatever the image holds, and no find can cycle.
+__global__ void k_ccl_from_labels(size_t npx, int count, const uint32_t* __restrict__ labels,
+                                  unsigned* __restrict__ P) {
+  const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t z = blockIdx.y;
+  if (idx >= npx || static_cast<int>(z) >= count) return;
+  const uint32_t* lz = labels + z * npx;
+  const uint32_t l = lz[idx];
+  const bool valid = l != 0u && l <= npx && lz[l - 1u] == l;
+  P[z * npx + idx] = valid ? l - 1u : kNoLabel;
+}
+
+// A box of one component of one slice: label = root offset + 1, coordinates 0-based
+// (i fast, j slow), inclusive.
+struct CclBox {
+  unsigned label;
+  int i0, j0, i1, j1;
+};
+
+// Pass 3a (boxes): every foreground pixel points P at its root; a root claims a box slot
+// (slot order is arbitrary) and initialises it. M[root] = slot.
+__global__ void k_ccl_roots(size_t npx, int count, unsigned* __restrict__ P, unsigned* __restrict__ M,
+                            CclBox* __restrict__ boxes, int max_boxes, int* __restrict__ nbox) {
+  const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t z = blockIdx.y;
+  if (idx >= npx || static_cast<int>(z) >= count) return;
+  unsigned* Ps = P + z * npx;
+  const unsigned p = Ps[idx];
+  if (p == kNoLabel) return;
+  const unsigned r = g_find(Ps, p);
+  if (r == idx) {
+    const int slot = atomicAdd(nbox + z, 1);
+    M[z * npx + idx] = static_cast<unsigned>(slot);
+    if (slot < max_boxes) {
+      CclBox b;
+      b.label = r + 1u;
+      b.i0 = 0x7fffffff;
+      b.j0 = 0x7fffffff;
+      b.i1 = -1;
+      b.j1 = -1;
+      boxes[z * max_boxes + slot] = b;
+    }
+  }
+  Ps[idx] = r;  // root of a root is itself; every foreground entry now points at its root
+}
+
+// Pass 3b (boxes): a thread walks kBoxRun consecutive pixels of a row and issues one set of
+// min / max atomics per run of equal roots.
+constexpr int kBoxRun = 16;
+__global__ void k_ccl_boxes(int nx, int ny, int count, const unsigned* __restrict__ P,
+                            const unsigned* __restrict__ M, CclBox* __restrict__ boxes, int max_boxes) {
+  const int segs = (nx + kBoxRun - 1) / kBoxRun;
+  const size_t g = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t z = blockIdx.y;
+  if (g >= static_cast<size_t>(segs) * ny || static_cast<int>(z) >= count) return;
+  const int j = static_cast<int>(g / segs);
+  const int i_begin = static_cast<int>(g % segs) * kBoxRun;
+  const int i_end = min(i_begin + kBoxRun, nx);
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const unsigned* Ps = P + z * npx + static_cast<size_t>(j) * nx;
+  const unsigned* Ms = M + z * npx;
+  CclBox* bz = boxes + z * max_boxes;
+  unsigned cur = kNoLabel;
+  int first = 0, last = 0;
+  auto flush = [&]() {
+    if (cur == kNoLabel) return;
+    const unsigned s = Ms[cur];
+    if (s >= static_cast<unsigned>(max_boxes)) return;
+    atomicMin(&bz[s].i0, first);
+    atomicMax(&bz[s].i1, last);
+    atomicMin(&bz[s].j0, j);
+    atomicMax(&bz[s].j1, j);
+  };
+  for (int i = i_begin; i < i_end; ++i) {
+    const unsigned r = Ps[i];
+    if (r != cur) {
+      flush();
+      cur = r;
+      first = i;
+    }
+    last = i;
+  }
+  flush();
+}
+
+// cu_dilate (particle_detection.jl:12-42): out = OR of the 3 x 3 in-plane neighbourhood for
+// interior pixels (1 <= i <= nx-2, 1 <= j <= ny-2), false on the border.
+__global__ void k_dilate3x3(int nx, int ny, int count, const uint8_t* __restrict__ in, uint8_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const size_t z = blockIdx.z;
+  if (i >= nx || j >= ny || static_cast<int>(z) >= count) return;
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const uint8_t* s = in + z * npx;
+  uint8_t v = 0;
+  if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) {
+    for (int dj = -1; dj <= 1; ++dj) {
+      const uint8_t* r = s + static_cast<size_t>(j + dj) * nx + i;
+      v |= static_cast<uint8_t>((r[-1] != 0) | (r[0] != 0) | (r[1] != 0));
+    }
+  }
+  out[z * npx + static_cast<size_t>(j) * nx + i] = v;
+}
+
+}  // namespace holo

@@ -20,6 +20,7 @@
 #include "holo.h"
 #include "holo_kernels.hpp"
 #include "ingest_kernels.hpp"
+#include "ccl_kernels.hpp"
 
 using namespace holo;
 
@@ -964,6 +965,157 @@ int holo_make_background(holo_handle h, void* stream, int nx, int ny, int count,
     return fail(HOLO_ERR_ARG, "kind must be HOLO_BACKGROUND_MODE or HOLO_BACKGROUND_MEAN");
   }
   return check_launch();
+}
+
+// ---- connected components, boxes, dilation (ccl.jl:186-264, particle_detection.jl:12-96) ----
+static_assert(sizeof(holo_box) == sizeof(CclBox), "holo_box mirrors CclBox");
+
+namespace {
+
+int ccl_source(int kind, const void* img, double threshold, CclSrc* src) {
+  src->img = img;
+  src->lim = 0;
+  if (kind == HOLO_CCL_F32) {
+    src->kind = CCL_F32;
+  } else if (kind == HOLO_CCL_BINARY) {
+    src->kind = CCL_BINARY;
+  } else if (kind == HOLO_CCL_N0F8_LE) {
+    // d_vol .<= threshold compares Float64(v/255) (a correctly rounded quotient) with the
+    // threshold; the foreground bytes are a prefix 0 .. lim-1
+    src->kind = CCL_N0F8_LE;
+    unsigned lim = 0;
+    while (lim < 256 && static_cast<double>(lim) / 255.0 <= threshold) ++lim;
+    src->lim = lim;
+  } else {
+    return fail(HOLO_ERR_TYPE, "kind must be HOLO_CCL_F32, HOLO_CCL_BINARY or HOLO_CCL_N0F8_LE");
+  }
+  return HOLO_OK;
+}
+
+// Slices per round so that the union-find state (P and the root slots M, 8 B/px) stays
+// within 256 MiB; at least one.
+int ccl_round(size_t npx, int count) {
+  const size_t per = 8 * npx;
+  size_t r = per ? (size_t(256) << 20) / per : 1;
+  if (r < 1) r = 1;
+  if (r > 65535) r = 65535;
+  return static_cast<int>(std::min<size_t>(r, static_cast<size_t>(count)));
+}
+
+int ccl_passes_1_2(int nx, int ny, int c, const CclSrc& src, unsigned* P, hipStream_t st) {
+  const dim3 grid((nx + kTileX - 1) / kTileX, (ny + kTileY - 1) / kTileY, c);
+  hipLaunchKernelGGL(k_ccl_local, grid, dim3(256), 0, st, nx, ny, src, P);
+  HOLO_TRY(check_launch());
+  hipLaunchKernelGGL(k_ccl_border, grid, dim3(128), 0, st, nx, ny, src, P);
+  return check_launch();
+}
+
+int ccl_boxes_round(int nx, int ny, int c, unsigned* P, unsigned* M, holo_box* boxes, int max_boxes,
+                    int* nbox, hipStream_t st) {
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  HOLO_TRY(launch_fill(reinterpret_cast<float*>(nbox), c, 0u, st));
+  const dim3 g3(static_cast<unsigned>((npx + 255) / 256), c);
+  hipLaunchKernelGGL(k_ccl_roots, g3, dim3(256), 0, st, npx, c, P, M, reinterpret_cast<CclBox*>(boxes),
+                     max_boxes, nbox);
+  HOLO_TRY(check_launch());
+  const size_t segs = static_cast<size_t>((nx + kBoxRun - 1) / kBoxRun) * ny;
+  hipLaunchKernelGGL(k_ccl_boxes, dim3(static_cast<unsigned>((segs + 255) / 256), c), dim3(256), 0, st,
+                     nx, ny, c, P, M, reinterpret_cast<CclBox*>(boxes), max_boxes);
+  return check_launch();
+}
+
+int ccl_check_dims(int nx, int ny, int count) {
+  if (nx < 1 || ny < 1 || count < 0) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  if (static_cast<unsigned long long>(nx) * ny > 0xfffffffeull)
+    return fail(HOLO_ERR_ARG, "Image is too large. Maximum length is 2^32-1.");
+  return HOLO_OK;
+}
+
+}  // namespace
+
+int holo_ccl_label(holo_handle h, void* stream, int nx, int ny, int count, const void* img, int kind,
+                   double threshold, uint32_t* labels) {
+  if (!h || !img || !labels) return fail(HOLO_ERR_ARG, "NULL argument");
+  HOLO_TRY(ccl_check_dims(nx, ny, count));
+  CclSrc src;
+  HOLO_TRY(ccl_source(kind, img, threshold, &src));
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const size_t esz = kind == HOLO_CCL_F32 ? 4 : 1;
+  const int round = ccl_round(npx, std::max(count, 1));
+  HOLO_TRY(ensure_ws(h, static_cast<size_t>(round) * npx * 4));
+  unsigned* P = static_cast<unsigned*>(h->ws);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (int z0 = 0; z0 < count; z0 += round) {
+    const int c = std::min(round, count - z0);
+    CclSrc s = src;
+    s.img = static_cast<const uint8_t*>(img) + static_cast<size_t>(z0) * npx * esz;
+    HOLO_TRY(ccl_passes_1_2(nx, ny, c, s, P, st));
+    hipLaunchKernelGGL(k_ccl_labels, dim3(static_cast<unsigned>((npx + 255) / 256), c), dim3(256), 0, st,
+                       npx, c, P, labels + static_cast<size_t>(z0) * npx);
+    HOLO_TRY(check_launch());
+  }
+  return HOLO_OK;
+}
+
+int holo_ccl_boxes(holo_handle h, void* stream, int nx, int ny, int count, const void* img, int kind,
+                   double threshold, int max_boxes, holo_box* boxes, int* nboxes) {
+  if (!h || !img || !boxes || !nboxes) return fail(HOLO_ERR_ARG, "NULL argument");
+  HOLO_TRY(ccl_check_dims(nx, ny, count));
+  if (max_boxes < 1) return fail(HOLO_ERR_ARG, "max_boxes must be positive");
+  CclSrc src;
+  HOLO_TRY(ccl_source(kind, img, threshold, &src));
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const size_t esz = kind == HOLO_CCL_F32 ? 4 : 1;
+  const int round = ccl_round(npx, std::max(count, 1));
+  HOLO_TRY(ensure_ws(h, static_cast<size_t>(round) * npx * 8));
+  unsigned* P = static_cast<unsigned*>(h->ws);
+  unsigned* M = P + static_cast<size_t>(round) * npx;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (int z0 = 0; z0 < count; z0 += round) {
+    const int c = std::min(round, count - z0);
+    CclSrc s = src;
+    s.img = static_cast<const uint8_t*>(img) + static_cast<size_t>(z0) * npx * esz;
+    HOLO_TRY(ccl_passes_1_2(nx, ny, c, s, P, st));
+    HOLO_TRY(ccl_boxes_round(nx, ny, c, P, M, boxes + static_cast<size_t>(z0) * max_boxes, max_boxes,
+                             nboxes + z0, st));
+  }
+  return HOLO_OK;
+}
+
+int holo_label_boxes(holo_handle h, void* stream, int nx, int ny, int count, const uint32_t* labels,
+                     int max_boxes, holo_box* boxes, int* nboxes) {
+  if (!h || !labels || !boxes || !nboxes) return fail(HOLO_ERR_ARG, "NULL argument");
+  HOLO_TRY(ccl_check_dims(nx, ny, count));
+  if (max_boxes < 1) return fail(HOLO_ERR_ARG, "max_boxes must be positive");
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const int round = ccl_round(npx, std::max(count, 1));
+  HOLO_TRY(ensure_ws(h, static_cast<size_t>(round) * npx * 8));
+  unsigned* P = static_cast<unsigned*>(h->ws);
+  unsigned* M = P + static_cast<size_t>(round) * npx;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (int z0 = 0; z0 < count; z0 += round) {
+    const int c = std::min(round, count - z0);
+    hipLaunchKernelGGL(k_ccl_from_labels, dim3(static_cast<unsigned>((npx + 255) / 256), c), dim3(256), 0, st,
+                       npx, c, labels + static_cast<size_t>(z0) * npx, P);
+    HOLO_TRY(check_launch());
+    HOLO_TRY(ccl_boxes_round(nx, ny, c, P, M, boxes + static_cast<size_t>(z0) * max_boxes, max_boxes,
+                             nboxes + z0, st));
+  }
+  return HOLO_OK;
+}
+
+int holo_dilate(holo_handle h, void* stream, int nx, int ny, int count, const uint8_t* in, uint8_t* out) {
+  if (!h || !in || !out) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (nx < 1 || ny < 1 || count < 0) return fail(HOLO_ERR_ARG, "sizes must be positive");
+  if (in == out) return fail(HOLO_ERR_ARG, "dilation cannot run in place");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (int z0 = 0; z0 < count; z0 += 65535) {
+    const int c = std::min(65535, count - z0);
+    const size_t off = static_cast<size_t>(z0) * nx * ny;
+    hipLaunchKernelGGL(k_dilate3x3, dim3((nx + 255) / 256, ny, c), dim3(256), 0, st, nx, ny, c, in + off, out + off);
+    HOLO_TRY(check_launch());
+  }
+  return HOLO_OK;
 }
 
 int holo_reconst_xyprojection(holo_handle h, void* stream, int nx, int ny, const void* wavefront,

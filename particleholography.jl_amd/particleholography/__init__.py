@@ -12,8 +12,13 @@ from .frequency_filters import *  # noqa: F401,F403
 from .frequency_filters import __all__ as _filters_all
 from .utils import *  # noqa: F401,F403
 from .utils import __all__ as _utils_all
+from .ccl import *  # noqa: F401,F403
+from .ccl import __all__ as _ccl_all
+from .particle_detection import *  # noqa: F401,F403
+from .particle_detection import __all__ as _detection_all
 
-__all__ = list(_holofunc_all) + list(_filters_all) + list(_utils_all) + ["reconst_xyprojection_holo"]
+__all__ = (list(_holofunc_all) + list(_filters_all) + list(_utils_all) + list(_ccl_all) + list(_detection_all)
+           + ["reconst_xyprojection_holo"])
 
 
 def native_library_path() -> str:

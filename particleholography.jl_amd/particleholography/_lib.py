@@ -17,6 +17,7 @@ HOLO_N0F8, HOLO_F16, HOLO_F32, HOLO_F64, HOLO_C64 = range(5)
 HOLO_TF_REFERENCE, HOLO_TF_FP64_D, HOLO_TF_FP64 = range(3)
 HOLO_LPF_RECTANGLE, HOLO_LPF_SUPER_GAUSSIAN = range(2)
 HOLO_BACKGROUND_MODE, HOLO_BACKGROUND_MEAN = range(2)
+HOLO_CCL_F32, HOLO_CCL_BINARY, HOLO_CCL_N0F8_LE = range(3)
 
 
 class ArgumentError(ValueError):
@@ -42,6 +43,11 @@ class holo_gs_job(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("holo1", ctypes.c_void_p),
                 ("holo2", ctypes.c_void_p), ("fwd", holo_tf), ("bwd", holo_tf),
                 ("iters", ctypes.c_int), ("out", ctypes.c_void_p)]
+
+
+class holo_box(ctypes.Structure):
+    _fields_ = [("label", ctypes.c_uint32), ("i0", ctypes.c_int32), ("j0", ctypes.c_int32),
+                ("i1", ctypes.c_int32), ("j1", ctypes.c_int32)]
 
 
 _vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
@@ -70,6 +76,10 @@ SIGNATURES = {
     "holo_apply_low_pass_filter": [_vp, _vp, _i, _i, _vp, _vp, _vp],
     "holo_ingest_u8": [_vp, _vp, _i, _i, _i, _vp, _vp, _d, _vp],
     "holo_make_background": [_vp, _vp, _i, _i, _i, _vp, _i, _vp],
+    "holo_ccl_label": [_vp, _vp, _i, _i, _i, _vp, _i, _d, _vp],
+    "holo_ccl_boxes": [_vp, _vp, _i, _i, _i, _vp, _i, _d, _i, _vp, _vp],
+    "holo_label_boxes": [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp],
+    "holo_dilate": [_vp, _vp, _i, _i, _i, _vp, _vp],
     "holo_reconst_batch": [_vp, _vp, ctypes.POINTER(holo_job), _i],
     "holo_phase_retrieval_batch": [_vp, _vp, ctypes.POINTER(holo_gs_job), _i],
     "holo_set_tuning_for_size": [_vp, _i, _i, _i, _i, _i, _i, _i],

@@ -1,0 +1,262 @@
+"""Connected components and particle neighbourhoods: ParticleHolography.jl's ccl.jl over libholo.
+
+Device work (libholo): ``cu_connected_component_labeling`` (ccl.jl:186-218),
+``cu_find_valid_labels`` and ``get_bounding_rectangles`` (ccl.jl:240-264) through the box
+kernels. The label of a pixel is 1 + the smallest memory offset of its 8-connected component
+of equal positive values, 0 for background -- the labels the reference's union-find leaves.
+
+Host work, as in the reference: the z-linking of per-slice rectangles into particle
+neighbourhoods (ccl.jl:266-466). Neighbourhoods are a dict from UUID to
+``[x_min, y_min, z_min, x_max, y_max, z_max]`` (1-based, x = Julia dim 2, y = Julia dim 1).
+Julia's Dict iterates in hash order of the UUID keys; this dict iterates in insertion order,
+and entries deleted while a loop runs are skipped as in Julia. The results agree wherever
+the reference's merging does not depend on iteration order.
+
+Layout: a Julia (height, width) label image is a tensor of shape (width, height) here (same
+memory), so a rectangle's x comes from the tensor's row index and y from its column index.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import random
+import uuid
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, holo_box
+
+__all__ = ["cu_connected_component_labeling", "count_labels", "cu_find_valid_labels",
+           "get_bounding_rectangles", "gen_particle_neighborhoods", "update_particle_neighborhoods_",
+           "update_particle_neighborhoods3d_", "finalize_particle_neighborhoods_", "judge_overlap2d",
+           "judge_overlap3d", "new_rect", "delete_duplicates_"]
+
+_TYPEMAX, _TYPEMIN = 2 ** 63 - 1, -(2 ** 63)
+
+
+def _dev(t: torch.Tensor) -> int:
+    if not t.is_cuda:
+        raise AssertionError("arrays must be device (GPU) tensors")
+    return t.device.index or 0
+
+
+def _kind_of(t: torch.Tensor) -> int:
+    if t.dtype == torch.float32:
+        return _lib.HOLO_CCL_F32
+    if t.dtype in (torch.bool, torch.uint8):
+        return _lib.HOLO_CCL_BINARY
+    raise AssertionError(f"images must be float32, bool or uint8, got {t.dtype}")
+
+
+def cu_connected_component_labeling(input_img: torch.Tensor) -> torch.Tensor:
+    """ccl.jl:202-227: 8-way labels (uint32, same shape) of a 2-D image (float32: equal
+    positive values connect; bool/uint8: nonzero is foreground). A 3-D tensor is labelled
+    slice by slice."""
+    assert input_img.numel() <= 2 ** 32 - 1, "Image is too large. Maximum length is 2^32-1."
+    img = input_img.contiguous()
+    device = _dev(img)
+    squeeze = img.dim() == 2
+    stack = img[None] if squeeze else img
+    count, ny, nx = stack.shape
+    labels = torch.empty(stack.shape, dtype=torch.uint32, device=img.device)
+    kind = _kind_of(stack)
+    src = stack.view(torch.uint8) if stack.dtype == torch.bool else stack
+    check(_lib.load().holo_ccl_label(_lib.handle(device).ptr, torch.cuda.current_stream(device).cuda_stream,
+                                     nx, ny, count, src.data_ptr(), kind, 0.0, labels.data_ptr()))
+    return labels[0] if squeeze else labels
+
+
+def count_labels(labels) -> int:
+    """ccl.jl:229-237: number of entries whose label is their own 1-based linear index."""
+    a = _host_u32(labels).reshape(-1)
+    return int(np.count_nonzero(a == np.arange(1, a.size + 1, dtype=np.uint64)))
+
+
+def _host_u32(labels) -> np.ndarray:
+    if isinstance(labels, torch.Tensor):
+        labels = labels.detach().cpu().numpy()
+    return np.asarray(labels).astype(np.uint64)
+
+
+def _boxes_of_labels(labels: torch.Tensor, max_boxes: int = 1 << 16) -> List[Tuple[int, int, int, int, int]]:
+    """(label, i0, j0, i1, j1) of every component of a device label image, by label."""
+    lab = labels.contiguous()
+    if lab.dtype != torch.uint32:
+        lab = lab.to(torch.int64).to(torch.uint32)
+    device = _dev(lab)
+    ny, nx = lab.shape
+    while True:
+        boxes = torch.empty((max_boxes, 5), dtype=torch.int32, device=lab.device)
+        nbox = torch.empty(1, dtype=torch.int32, device=lab.device)
+        check(_lib.load().holo_label_boxes(_lib.handle(device).ptr, torch.cuda.current_stream(device).cuda_stream,
+                                           nx, ny, 1, lab.data_ptr(), max_boxes, boxes.data_ptr(),
+                                           nbox.data_ptr()))
+        n = int(nbox.item())
+        if n <= max_boxes:
+            break
+        max_boxes = n
+    b = boxes[:n].cpu().numpy()
+    b = b[np.argsort(b[:, 0].view(np.uint32), kind="stable")]
+    return [(int(np.uint32(r[0])), int(r[1]), int(r[2]), int(r[3]), int(r[4])) for r in b]
+
+
+def cu_find_valid_labels(labels: torch.Tensor) -> List[int]:
+    """ccl.jl:250-254: the 1-based linear indices whose label equals the index (one per
+    component), ascending."""
+    return [b[0] for b in _boxes_of_labels(labels)]
+
+
+def get_bounding_rectangles(labels, valid_labels: Sequence[int]) -> List[Tuple[int, int, int, int]]:
+    """ccl.jl:256-281: (x_min, y_min, x_max, y_max) of each label of ``valid_labels``, in that
+    order, 1-based with x the Julia column (tensor row) and y the Julia row (tensor column);
+    a label without pixels gives (typemax, typemax, typemin, typemin)."""
+    lab = labels if isinstance(labels, torch.Tensor) else torch.from_numpy(np.asarray(labels, np.uint32))
+    if not lab.is_cuda:
+        lab = lab.to("cuda")
+    found = {b[0]: b for b in _boxes_of_labels(lab)}
+    out = []
+    for l in valid_labels:
+        b = found.get(int(l))
+        if b is None:
+            out.append((_TYPEMAX, _TYPEMAX, _TYPEMIN, _TYPEMIN))
+        else:
+            _, i0, j0, i1, j1 = b
+            out.append((j0 + 1, i0 + 1, j1 + 1, i1 + 1))
+    return out
+
+
+# ------------------------------------------------------------------------------------------
+# particle neighbourhoods (ccl.jl:266-466), host logic
+# ------------------------------------------------------------------------------------------
+def _uuid_source():
+    rng = random.Random(1234)  # the reference seeds a fresh MersenneTwister(1234) per call
+    return lambda: uuid.uuid1(node=rng.getrandbits(48) | (1 << 40), clock_seq=rng.getrandbits(14))
+
+
+def gen_particle_neighborhoods(bounding_rectangles, slicenum: int) -> Dict[uuid.UUID, List[int]]:
+    """ccl.jl:266-277."""
+    new_key = _uuid_source()
+    out: Dict[uuid.UUID, List[int]] = {}
+    for x_min, y_min, x_max, y_max in bounding_rectangles:
+        out[new_key()] = [x_min, y_min, slicenum, x_max, y_max, slicenum]
+    return out
+
+
+def judge_overlap2d(rect1, rect2) -> bool:
+    """ccl.jl:279-288 (strict inequalities)."""
+    x_min1, y_min1, x_max1, y_max1 = rect1
+    x_min2, y_min2, x_max2, y_max2 = rect2
+    return x_min1 < x_max2 and x_max1 > x_min2 and y_min1 < y_max2 and y_max1 > y_min2
+
+
+def judge_overlap3d(rect1, rect2) -> bool:
+    """ccl.jl:290-299."""
+    x_min1, y_min1, z_min1, x_max1, y_max1, z_max1 = rect1
+    x_min2, y_min2, z_min2, x_max2, y_max2, z_max2 = rect2
+    return (x_min1 < x_max2 and x_max1 > x_min2 and y_min1 < y_max2 and y_max1 > y_min2
+            and z_min1 < z_max2 and z_max1 > z_min2)
+
+
+def new_rect(rect1, rect2) -> List[int]:
+    """ccl.jl:301-312."""
+    x_min1, y_min1, x_max1, y_max1 = rect1
+    x_min2, y_min2, x_max2, y_max2 = rect2
+    return [min(x_min1, x_min2), min(y_min1, y_min2), max(x_max1, x_max2), max(y_max1, y_max2)]
+
+
+def _drop_unit_cubes(pn) -> None:
+    for k in list(pn):
+        v = pn[k]
+        if abs(v[0] - v[3]) == 1 and abs(v[1] - v[4]) == 1 and abs(v[2] - v[5]) == 1:
+            del pn[k]
+
+
+def update_particle_neighborhoods_(particle_neighborhoods, bounding_rectangles, slicenum: int) -> None:
+    """ccl.jl:328-363 (update_particle_neighborhoods!): a rectangle overlapping a neighbourhood
+    in x-y grows it and extends its z-range; otherwise it starts a new neighbourhood."""
+    new_key = _uuid_source()
+    pn = particle_neighborhoods
+    for br in bounding_rectangles:
+        overlap = False
+        for v in list(pn.values()):
+            if judge_overlap2d(br, (v[0], v[1], v[3], v[4])):
+                nb = new_rect(br, (v[0], v[1], v[3], v[4]))
+                v[0], v[1], v[3], v[4] = nb
+                if slicenum < v[2]:
+                    v[2] = slicenum
+                elif slicenum > v[5]:
+                    v[5] = slicenum
+                overlap = True
+        if not overlap:
+            pn[new_key()] = [br[0], br[1], slicenum, br[2], br[3], slicenum]
+    _drop_unit_cubes(pn)
+
+
+def update_particle_neighborhoods3d_(particle_neighborhoods, bounding_rectangles, slicenum: int) -> None:
+    """ccl.jl:378-409 (update_particle_neighborhoods3d!): only neighbourhoods that end on the
+    previous slice are extended."""
+    new_key = _uuid_source()
+    pn = particle_neighborhoods
+    sub = [v for v in pn.values() if v[-1] == slicenum - 1]
+    for br in bounding_rectangles:
+        overlap = False
+        for v in sub:
+            if judge_overlap2d(br, (v[0], v[1], v[3], v[4])):
+                nb = new_rect(br, (v[0], v[1], v[3], v[4]))
+                v[0], v[1], v[3], v[4] = nb
+                v[5] = slicenum
+                overlap = True
+        if not overlap:
+            pn[new_key()] = [br[0], br[1], slicenum, br[2], br[3], slicenum]
+    _drop_unit_cubes(pn)
+
+
+def delete_duplicates_(particle_neighborhoods) -> None:
+    """ccl.jl:446-466 (delete_duplicates!): merges 3-D-overlapping neighbourhoods until none
+    overlap."""
+    pn = particle_neighborhoods
+    changed = True
+    while changed:
+        changed = False
+        for k in list(pn):
+            if k not in pn:
+                continue
+            v = pn[k]
+            for k2 in list(pn):
+                if k2 == k or k2 not in pn:
+                    continue
+                v2 = pn[k2]
+                if judge_overlap3d(tuple(v), tuple(v2)):
+                    nb = new_rect((v[0], v[1], v[3], v[4]), (v2[0], v2[1], v2[3], v2[4]))
+                    v[0], v[1], v[3], v[4] = nb
+                    v[2] = min(v[2], v2[2])
+                    v[5] = max(v[5], v2[5])
+                    del pn[k2]
+                    changed = True
+                    break
+
+
+def _ratio(a: int, b: int) -> float:
+    if b != 0:
+        return a / b
+    return math.inf if a > 0 else math.nan  # Julia Int / Int: x/0 = Inf, 0/0 = NaN
+
+
+def finalize_particle_neighborhoods_(particle_neighborhoods) -> None:
+    """ccl.jl:411-444 (finalize_particle_neighborhoods!): merges duplicates, then drops
+    neighbourhoods two slices deep, with an x/y extent ratio above 3 or below 1/3, or an x-y
+    extent product below 10."""
+    pn = particle_neighborhoods
+    delete_duplicates_(pn)
+    for k in list(pn):
+        v = pn[k]
+        if abs(v[2] - v[5]) == 1:
+            pn.pop(k, None)
+        r = _ratio(abs(v[0] - v[3]), abs(v[1] - v[4]))
+        if r > 3 or r < 1 / 3:
+            pn.pop(k, None)
+        if abs(v[0] - v[3]) * abs(v[1] - v[4]) < 10:
+            pn.pop(k, None)
