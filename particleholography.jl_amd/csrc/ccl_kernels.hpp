@@ -7,10 +7,11 @@
 // pixels are 0: the labels the reference's union-find produces (every union links the
 // larger root under the smaller, so the surviving root is the first pixel in memory order).
 //
-// Three passes over a slice:
-//   1. k_ccl_local: a workgroup owns a 64 x 16 tile, unions its pixels in LDS and writes for
-//      every foreground pixel the global offset of its tile-local root (P), background
-//      0xffffffff;
+// Three passes over a slice, all on a grid of 64 x 16 tiles:
+//   1. k_ccl_local: a workgroup unions its tile's pixels in LDS and writes for every
+//      foreground pixel the global offset of its tile-local root (P), background 0xffffffff;
+//      a tile without foreground writes only its flag (0), and later passes skip it, so a
+//      sparse volume costs one read of the image;
 //   2. k_ccl_border: unions across tile edges only (W and NW of the left column, NW/N/NE of
 //      the top row, NE of the right column), lock-free on P with atomicMin;
 //   3. k_ccl_labels (labels = root + 1), or k_ccl_roots + k_ccl_boxes (one box slot per root,
@@ -38,6 +39,7 @@ struct CclSrc {
   const void* img;   // slice z at img + z * npx elements
   int kind;
   unsigned lim;      // CCL_N0F8_LE: number of foreground byte values
+  int vec;           // nx % 4 == 0 and the image aligned for 4-pixel loads
 };
 
 __device__ __forceinline__ unsigned ccl_key(const CclSrc& s, size_t idx) {
@@ -107,59 +109,104 @@ __device__ __forceinline__ void g_union(unsigned* P, unsigned a, unsigned b) {
   }
 }
 
-// Pass 1. grid (ceil(nx/64), ceil(ny/16), count), 256 threads: thread t owns column t % 64 of
-// rows t/64 + 4k, k = 0..3.
-__global__ __launch_bounds__(256) void k_ccl_local(int nx, int ny, CclSrc src, unsigned* __restrict__ P) {
-  __shared__ unsigned key[kTilePx];
-  __shared__ unsigned par[kTilePx];
-  const int t = threadIdx.x;
-  const int lx = t % kTileX;
-  const int x = blockIdx.x * kTileX + lx;
-  const int y0 = blockIdx.y * kTileY;
-  const size_t npx = static_cast<size_t>(nx) * ny;
-  const size_t base = static_cast<size_t>(blockIdx.z) * npx;
+// Thread t of a 256-thread tile workgroup owns the 4 adjacent pixels lx = 4*(t % 16) + c,
+// c = 0..3, of row ly = t / 16. `vec` (host-checked: nx % 4 == 0 and 4-byte / 16-byte aligned
+// image) lets the 4 keys load as one uchar4 / float4 and P store as one uint4.
+__device__ __forceinline__ void tile_keys(const CclSrc& s, size_t base, int nx, int ny, int x0, int y,
+                                          int vec, unsigned k4[4]) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int ly = t / kTileX + 4 * k;
-    const int y = y0 + ly;
-    const int i = ly * kTileX + lx;
-    key[i] = (x < nx && y < ny) ? ccl_key(src, base + static_cast<size_t>(y) * nx + x) : 0u;
-    par[i] = i;
-  }
-  __syncthreads();
+  for (int c = 0; c < 4; ++c) k4[c] = 0u;
+  if (y >= ny) return;
+  const size_t o = base + static_cast<size_t>(y) * nx + x0;
+  if (vec && x0 + 3 < nx) {
+    if (s.kind == CCL_F32) {
+      const float4 w = *reinterpret_cast<const float4*>(static_cast<const float*>(s.img) + o);
+      const float f[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int ly = t / kTileX + 4 * k;
-    const int i = ly * kTileX + lx;
-    const unsigned c = key[i];
-    if (c == 0u) continue;
-    if (lx > 0 && key[i - 1] == c) lds_union(par, i, i - 1);
-    if (ly > 0) {
-      if (lx > 0 && key[i - kTileX - 1] == c) lds_union(par, i, i - kTileX - 1);
-      if (key[i - kTileX] == c) lds_union(par, i, i - kTileX);
-      if (lx < kTileX - 1 && key[i - kTileX + 1] == c) lds_union(par, i, i - kTileX + 1);
+      for (int c = 0; c < 4; ++c) k4[c] = f[c] > 0.0f ? __float_as_uint(f[c]) : 0u;
+    } else {
+      const uchar4 w = *reinterpret_cast<const uchar4*>(static_cast<const uint8_t*>(s.img) + o);
+      const unsigned b[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        k4[c] = s.kind == CCL_BINARY ? (b[c] != 0u ? 1u : 0u) : (b[c] < s.lim ? 1u : 0u);
     }
-  }
-  __syncthreads();
+  } else {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int ly = t / kTileX + 4 * k;
-    const int y = y0 + ly;
-    if (x >= nx || y >= ny) continue;
-    const int i = ly * kTileX + lx;
-    unsigned out = kNoLabel;
-    if (key[i] != 0u) {
-      const unsigned r = lds_find(par, i);
-      out = static_cast<unsigned>((y0 + static_cast<int>(r) / kTileX) * nx + blockIdx.x * kTileX + r % kTileX);
-    }
-    P[base + static_cast<size_t>(y) * nx + x] = out;
+    for (int c = 0; c < 4; ++c)
+      if (x0 + c < nx) k4[c] = ccl_key(s, o + c);
   }
 }
 
-// Pass 2. grid (ceil(nx/64), ceil(ny/16), count), 128 threads: t < 64 the top-row pixel
-// lx = t (NW, N, NE across the top edge), 64 <= t < 80 the left-column pixel ly = t - 64
-// (W, NW), 80 <= t < 96 the right-column pixel ly = t - 80 (NE).
-__global__ __launch_bounds__(128) void k_ccl_border(int nx, int ny, CclSrc src, unsigned* __restrict__ P) {
+// Pass 1. grid (ceil(nx/64), ceil(ny/16), count). A tile without foreground only clears its
+// flag; otherwise the tile's pixels are unioned in LDS, P receives each foreground pixel's
+// tile-local root (background kNoLabel) and the flag is set.
+__global__ __launch_bounds__(256) void k_ccl_local(int nx, int ny, CclSrc src, unsigned* __restrict__ P,
+                                                   unsigned* __restrict__ flags) {
+  __shared__ unsigned key[kTilePx];
+  __shared__ unsigned par[kTilePx];
+  const int t = threadIdx.x;
+  const int ly = t >> 4, lx0 = 4 * (t & 15);
+  const int x0 = blockIdx.x * kTileX + lx0;
+  const int y0 = blockIdx.y * kTileY;
+  const int y = y0 + ly;
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const size_t base = static_cast<size_t>(blockIdx.z) * npx;
+  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  unsigned k4[4];
+  tile_keys(src, base, nx, ny, x0, y, src.vec, k4);
+  const int i0 = ly * kTileX + lx0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    key[i0 + c] = k4[c];
+    par[i0 + c] = i0 + c;
+  }
+  const int any = __syncthreads_or((k4[0] | k4[1] | k4[2] | k4[3]) != 0u);
+  if (!any) {
+    if (t == 0) flags[tile] = 0u;
+    return;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int lx = lx0 + c, i = i0 + c;
+    const unsigned k = k4[c];
+    if (k == 0u) continue;
+    if (lx > 0 && key[i - 1] == k) lds_union(par, i, i - 1);
+    if (ly > 0) {
+      if (lx > 0 && key[i - kTileX - 1] == k) lds_union(par, i, i - kTileX - 1);
+      if (key[i - kTileX] == k) lds_union(par, i, i - kTileX);
+      if (lx < kTileX - 1 && key[i - kTileX + 1] == k) lds_union(par, i, i - kTileX + 1);
+    }
+  }
+  __syncthreads();
+  if (t == 0) flags[tile] = 1u;
+  if (y >= ny) return;
+  unsigned out[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    out[c] = kNoLabel;
+    if (k4[c] != 0u) {
+      const unsigned r = lds_find(par, i0 + c);
+      out[c] = static_cast<unsigned>((y0 + static_cast<int>(r) / kTileX) * nx + blockIdx.x * kTileX + r % kTileX);
+    }
+  }
+  unsigned* dst = P + base + static_cast<size_t>(y) * nx + x0;
+  if (src.vec && x0 + 3 < nx) {
+    *reinterpret_cast<uint4*>(dst) = make_uint4(out[0], out[1], out[2], out[3]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (x0 + c < nx) dst[c] = out[c];
+  }
+}
+
+// Pass 2. Same grid, 128 threads: t < 64 the top-row pixel lx = t (NW, N, NE across the top
+// edge), 64 <= t < 80 the left-column pixel ly = t - 64 (W, NW), 80 <= t < 96 the right-column
+// pixel ly = t - 80 (NE). Tiles without foreground have no edges to join.
+__global__ __launch_bounds__(128) void k_ccl_border(int nx, int ny, CclSrc src, unsigned* __restrict__ P,
+                                                    const unsigned* __restrict__ flags) {
+  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  if (flags[tile] == 0u) return;
   const int t = threadIdx.x;
   const int x0 = blockIdx.x * kTileX, y0 = blockIdx.y * kTileY;
   int x, y;
@@ -201,26 +248,47 @@ __global__ __launch_bounds__(128) void k_ccl_border(int nx, int ny, CclSrc src, 
   }
 }
 
-// Pass 3 (labels): labels = root + 1 for foreground pixels, 0 for background. One thread
-// per pixel; P is compressed to the root on the way.
-__global__ void k_ccl_labels(size_t npx, int count, unsigned* __restrict__ P, uint32_t* __restrict__ labels) {
-  const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t z = blockIdx.y;
-  if (idx >= npx || static_cast<int>(z) >= count) return;
-  unsigned* Ps = P + z * npx;
-  const unsigned p = Ps[idx];
-  uint32_t out = 0u;
-  if (p != kNoLabel) {
-    const unsigned r = g_find(Ps, p);
-    Ps[idx] = r;
-    out = r + 1u;
+// Pass 3 (labels), tile grid: labels = root + 1 for foreground pixels, 0 for background (a
+// whole empty tile is written with zeros without reading P). P is compressed on the way.
+__global__ __launch_bounds__(256) void k_ccl_labels(int nx, int ny, unsigned* __restrict__ P,
+                                                    const unsigned* __restrict__ flags,
+                                                    uint32_t* __restrict__ labels, int vec) {
+  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  const int t = threadIdx.x;
+  const int x0 = blockIdx.x * kTileX + 4 * (t & 15);
+  const int y = blockIdx.y * kTileY + (t >> 4);
+  if (y >= ny) return;
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const size_t row = static_cast<size_t>(blockIdx.z) * npx + static_cast<size_t>(y) * nx;
+  unsigned out[4] = {0u, 0u, 0u, 0u};
+  if (flags[tile] != 0u) {
+    unsigned* Ps = P + static_cast<size_t>(blockIdx.z) * npx;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (x0 + c >= nx) continue;
+      const size_t idx = static_cast<size_t>(y) * nx + x0 + c;
+      const unsigned p = Ps[idx];
+      if (p != kNoLabel) {
+        const unsigned r = g_find(Ps, p);
+        Ps[idx] = r;
+        out[c] = r + 1u;
+      }
+    }
   }
-  labels[z * npx + idx] = out;
+  uint32_t* dst = labels + row + x0;
+  if (vec && x0 + 3 < nx) {
+    *reinterpret_cast<uint4*>(dst) = make_uint4(out[0], out[1], out[2], out[3]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (x0 + c < nx) dst[c] = out[c];
+  }
 }
 
 // P from a label image. A pixel joins label l only when l is a valid label -- the pixel at
 // offset l - 1 carries l itself (cu_find_valid_labels' test, ccl.jl:240-248) -- so every entry
 // of P is a self-pointing root or kNoLabel, whatever the image holds, and no find can cycle.
+// Every tile is flagged (the label image is not scanned for empty tiles).
 __global__ void k_ccl_from_labels(size_t npx, int count, const uint32_t* __restrict__ labels,
                                   unsigned* __restrict__ P) {
   const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -239,45 +307,57 @@ struct CclBox {
   int i0, j0, i1, j1;
 };
 
-// Pass 3a (boxes): every foreground pixel points P at its root; a root claims a box slot
-// (slot order is arbitrary) and initialises it. M[root] = slot.
-__global__ void k_ccl_roots(size_t npx, int count, unsigned* __restrict__ P, unsigned* __restrict__ M,
-                            CclBox* __restrict__ boxes, int max_boxes, int* __restrict__ nbox) {
-  const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t z = blockIdx.y;
-  if (idx >= npx || static_cast<int>(z) >= count) return;
+// Pass 3a (boxes), tile grid: every foreground pixel points P at its root; a root claims a
+// box slot (slot order is arbitrary) and initialises it; M[root] = slot.
+__global__ __launch_bounds__(256) void k_ccl_roots(int nx, int ny, unsigned* __restrict__ P,
+                                                   const unsigned* __restrict__ flags, unsigned* __restrict__ M,
+                                                   CclBox* __restrict__ boxes, int max_boxes, int* __restrict__ nbox) {
+  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  if (flags[tile] == 0u) return;
+  const int t = threadIdx.x;
+  const int x0 = blockIdx.x * kTileX + 4 * (t & 15);
+  const int y = blockIdx.y * kTileY + (t >> 4);
+  if (y >= ny) return;
+  const size_t z = blockIdx.z;
+  const size_t npx = static_cast<size_t>(nx) * ny;
   unsigned* Ps = P + z * npx;
-  const unsigned p = Ps[idx];
-  if (p == kNoLabel) return;
-  const unsigned r = g_find(Ps, p);
-  if (r == idx) {
-    const int slot = atomicAdd(nbox + z, 1);
-    M[z * npx + idx] = static_cast<unsigned>(slot);
-    if (slot < max_boxes) {
-      CclBox b;
-      b.label = r + 1u;
-      b.i0 = 0x7fffffff;
-      b.j0 = 0x7fffffff;
-      b.i1 = -1;
-      b.j1 = -1;
-      boxes[z * max_boxes + slot] = b;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (x0 + c >= nx) continue;
+    const unsigned idx = static_cast<unsigned>(y) * nx + x0 + c;
+    const unsigned p = Ps[idx];
+    if (p == kNoLabel) continue;
+    const unsigned r = g_find(Ps, p);
+    if (r == idx) {
+      const int slot = atomicAdd(nbox + z, 1);
+      M[z * npx + idx] = static_cast<unsigned>(slot);
+      if (slot < max_boxes) {
+        CclBox b;
+        b.label = r + 1u;
+        b.i0 = 0x7fffffff;
+        b.j0 = 0x7fffffff;
+        b.i1 = -1;
+        b.j1 = -1;
+        boxes[z * max_boxes + slot] = b;
+      }
     }
+    Ps[idx] = r;  // every foreground entry now points at its root
   }
-  Ps[idx] = r;  // root of a root is itself; every foreground entry now points at its root
 }
 
-// Pass 3b (boxes): a thread walks kBoxRun consecutive pixels of a row and issues one set of
-// min / max atomics per run of equal roots.
-constexpr int kBoxRun = 16;
-__global__ void k_ccl_boxes(int nx, int ny, int count, const unsigned* __restrict__ P,
-                            const unsigned* __restrict__ M, CclBox* __restrict__ boxes, int max_boxes) {
-  const int segs = (nx + kBoxRun - 1) / kBoxRun;
-  const size_t g = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t z = blockIdx.y;
-  if (g >= static_cast<size_t>(segs) * ny || static_cast<int>(z) >= count) return;
-  const int j = static_cast<int>(g / segs);
-  const int i_begin = static_cast<int>(g % segs) * kBoxRun;
-  const int i_end = min(i_begin + kBoxRun, nx);
+// Pass 3b (boxes), tile grid: a thread's 4 pixels issue one set of min / max atomics per run
+// of equal roots.
+__global__ __launch_bounds__(256) void k_ccl_boxes(int nx, int ny, const unsigned* __restrict__ P,
+                                                   const unsigned* __restrict__ flags,
+                                                   const unsigned* __restrict__ M, CclBox* __restrict__ boxes,
+                                                   int max_boxes) {
+  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  if (flags[tile] == 0u) return;
+  const int t = threadIdx.x;
+  const int x0 = blockIdx.x * kTileX + 4 * (t & 15);
+  const int j = blockIdx.y * kTileY + (t >> 4);
+  if (j >= ny) return;
+  const size_t z = blockIdx.z;
   const size_t npx = static_cast<size_t>(nx) * ny;
   const unsigned* Ps = P + z * npx + static_cast<size_t>(j) * nx;
   const unsigned* Ms = M + z * npx;
@@ -293,7 +373,8 @@ __global__ void k_ccl_boxes(int nx, int ny, int count, const unsigned* __restric
     atomicMin(&bz[s].j0, j);
     atomicMax(&bz[s].j1, j);
   };
-  for (int i = i_begin; i < i_end; ++i) {
+  const int i_end = min(x0 + 4, nx);
+  for (int i = x0; i < i_end; ++i) {
     const unsigned r = Ps[i];
     if (r != cur) {
       flush();

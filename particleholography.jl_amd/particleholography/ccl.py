@@ -32,7 +32,7 @@ from ._lib import check, holo_box
 __all__ = ["cu_connected_component_labeling", "count_labels", "cu_find_valid_labels",
            "get_bounding_rectangles", "gen_particle_neighborhoods", "update_particle_neighborhoods_",
            "update_particle_neighborhoods3d_", "finalize_particle_neighborhoods_", "judge_overlap2d",
-           "judge_overlap3d", "new_rect", "delete_duplicates_"]
+           "judge_overlap3d", "new_rect", "delete_duplicates_", "link_slices"]
 
 _TYPEMAX, _TYPEMIN = 2 ** 63 - 1, -(2 ** 63)
 
@@ -167,76 +167,144 @@ def new_rect(rect1, rect2) -> List[int]:
     return [min(x_min1, x_min2), min(y_min1, y_min2), max(x_max1, x_max2), max(y_max1, y_max2)]
 
 
-def _drop_unit_cubes(pn) -> None:
-    for k in list(pn):
-        v = pn[k]
-        if abs(v[0] - v[3]) == 1 and abs(v[1] - v[4]) == 1 and abs(v[2] - v[5]) == 1:
-            del pn[k]
+class _Table:
+    """The neighbourhoods of a dict as an (n, 6) int64 array in dict order, with appends and
+    deletions, written back to the dict (values updated in place) by ``store``."""
+
+    def __init__(self, pn):
+        self.pn = pn
+        self.keys = list(pn)
+        n = len(self.keys)
+        self.a = np.zeros((max(16, 2 * n), 6), np.int64)
+        if n:
+            self.a[:n] = np.array([pn[k] for k in self.keys], np.int64).reshape(n, 6)
+        self.n = n
+        self.dead = np.zeros(self.a.shape[0], bool)
+
+    def append(self, key, row):
+        if self.n == self.a.shape[0]:
+            self.a = np.concatenate([self.a, np.zeros_like(self.a)])
+            self.dead = np.concatenate([self.dead, np.zeros_like(self.dead)])
+        self.a[self.n] = row
+        self.keys.append(key)
+        self.n += 1
+
+    def store(self):
+        pn = self.pn
+        for idx, k in enumerate(self.keys):
+            if self.dead[idx]:
+                pn.pop(k, None)
+            elif k in pn:
+                pn[k][:] = [int(v) for v in self.a[idx]]
+            else:
+                pn[k] = [int(v) for v in self.a[idx]]
+
+
+def _overlap2d_rows(br, a):
+    """judge_overlap2d(br, (x_min, y_min, x_max, y_max) of each row)."""
+    return (br[0] < a[:, 3]) & (br[2] > a[:, 0]) & (br[1] < a[:, 4]) & (br[3] > a[:, 1])
+
+
+def _drop_unit_cubes(t: _Table) -> None:
+    a = t.a[:t.n]
+    unit = (np.abs(a[:, 0] - a[:, 3]) == 1) & (np.abs(a[:, 1] - a[:, 4]) == 1) & (np.abs(a[:, 2] - a[:, 5]) == 1)
+    t.dead[:t.n] |= unit
+
+
+def _update_table(t: _Table, bounding_rectangles, slicenum: int, three_d: bool, new_key) -> None:
+    """One call of update_particle_neighborhoods! (three_d False) or
+    update_particle_neighborhoods3d! (True) on a table."""
+    s = int(slicenum)
+    if three_d:
+        sub = np.nonzero((t.a[:t.n, 5] == s - 1) & ~t.dead[:t.n])[0]
+    for br in bounding_rectangles:
+        if three_d:
+            rows = t.a[sub]
+            m = _overlap2d_rows(br, rows)
+            if m.any():
+                r = rows[m]
+                r[:, 0] = np.minimum(r[:, 0], br[0])
+                r[:, 1] = np.minimum(r[:, 1], br[1])
+                r[:, 3] = np.maximum(r[:, 3], br[2])
+                r[:, 4] = np.maximum(r[:, 4], br[3])
+                r[:, 5] = s
+                t.a[sub[m]] = r
+                continue
+        else:
+            a = t.a[:t.n]
+            m = _overlap2d_rows(br, a) & ~t.dead[:t.n]
+            if m.any():
+                rows = a[m]
+                rows[:, 0] = np.minimum(rows[:, 0], br[0])
+                rows[:, 1] = np.minimum(rows[:, 1], br[1])
+                rows[:, 3] = np.maximum(rows[:, 3], br[2])
+                rows[:, 4] = np.maximum(rows[:, 4], br[3])
+                lt = s < rows[:, 2]
+                gt = ~lt & (s > rows[:, 5])
+                rows[lt, 2] = s
+                rows[gt, 5] = s
+                a[m] = rows
+                continue
+        t.append(new_key(), [br[0], br[1], s, br[2], br[3], s])
+    _drop_unit_cubes(t)
 
 
 def update_particle_neighborhoods_(particle_neighborhoods, bounding_rectangles, slicenum: int) -> None:
     """ccl.jl:328-363 (update_particle_neighborhoods!): a rectangle overlapping a neighbourhood
-    in x-y grows it and extends its z-range; otherwise it starts a new neighbourhood."""
-    new_key = _uuid_source()
-    pn = particle_neighborhoods
-    for br in bounding_rectangles:
-        overlap = False
-        for v in list(pn.values()):
-            if judge_overlap2d(br, (v[0], v[1], v[3], v[4])):
-                nb = new_rect(br, (v[0], v[1], v[3], v[4]))
-                v[0], v[1], v[3], v[4] = nb
-                if slicenum < v[2]:
-                    v[2] = slicenum
-                elif slicenum > v[5]:
-                    v[5] = slicenum
-                overlap = True
-        if not overlap:
-            pn[new_key()] = [br[0], br[1], slicenum, br[2], br[3], slicenum]
-    _drop_unit_cubes(pn)
+    in x-y grows it and extends its z-range; otherwise it starts a new neighbourhood. The
+    rectangles are taken in order, each against every neighbourhood present at that moment
+    (those started by earlier rectangles included); per rectangle the neighbourhoods do not
+    interact, so that scan is one vector operation."""
+    t = _Table(particle_neighborhoods)
+    _update_table(t, bounding_rectangles, slicenum, False, _uuid_source())
+    t.store()
 
 
 def update_particle_neighborhoods3d_(particle_neighborhoods, bounding_rectangles, slicenum: int) -> None:
     """ccl.jl:378-409 (update_particle_neighborhoods3d!): only neighbourhoods that end on the
-    previous slice are extended."""
-    new_key = _uuid_source()
-    pn = particle_neighborhoods
-    sub = [v for v in pn.values() if v[-1] == slicenum - 1]
-    for br in bounding_rectangles:
-        overlap = False
-        for v in sub:
-            if judge_overlap2d(br, (v[0], v[1], v[3], v[4])):
-                nb = new_rect(br, (v[0], v[1], v[3], v[4]))
-                v[0], v[1], v[3], v[4] = nb
-                v[5] = slicenum
-                overlap = True
-        if not overlap:
-            pn[new_key()] = [br[0], br[1], slicenum, br[2], br[3], slicenum]
-    _drop_unit_cubes(pn)
+    previous slice when the call starts are extended."""
+    t = _Table(particle_neighborhoods)
+    _update_table(t, bounding_rectangles, slicenum, True, _uuid_source())
+    t.store()
+
+
+def link_slices(per_slice_rectangles, three_d: bool = False):
+    """gen_particle_neighborhoods on slice 1, then one update per further slice, then
+    finalize_particle_neighborhoods! (particle_detection.jl:58-96), on one table."""
+    if not per_slice_rectangles:
+        return {}
+    pn = gen_particle_neighborhoods(per_slice_rectangles[0], 1)
+    t = _Table(pn)
+    for idx in range(2, len(per_slice_rectangles) + 1):
+        _update_table(t, per_slice_rectangles[idx - 1], idx, three_d, _uuid_source())
+    t.store()
+    finalize_particle_neighborhoods_(pn)
+    return pn
 
 
 def delete_duplicates_(particle_neighborhoods) -> None:
-    """ccl.jl:446-466 (delete_duplicates!): merges 3-D-overlapping neighbourhoods until none
-    overlap."""
-    pn = particle_neighborhoods
+    """ccl.jl:446-466 (delete_duplicates!): each neighbourhood in turn absorbs the first other
+    neighbourhood (in dict order) it overlaps in 3-D; passes repeat until none overlap."""
+    t = _Table(particle_neighborhoods)
     changed = True
     while changed:
         changed = False
-        for k in list(pn):
-            if k not in pn:
+        for k in range(t.n):
+            if t.dead[k]:
                 continue
-            v = pn[k]
-            for k2 in list(pn):
-                if k2 == k or k2 not in pn:
-                    continue
-                v2 = pn[k2]
-                if judge_overlap3d(tuple(v), tuple(v2)):
-                    nb = new_rect((v[0], v[1], v[3], v[4]), (v2[0], v2[1], v2[3], v2[4]))
-                    v[0], v[1], v[3], v[4] = nb
-                    v[2] = min(v[2], v2[2])
-                    v[5] = max(v[5], v2[5])
-                    del pn[k2]
-                    changed = True
-                    break
+            alive = np.nonzero(~t.dead[:t.n])[0]
+            b = t.a[alive]
+            v = t.a[k]
+            m = ((v[0] < b[:, 3]) & (v[3] > b[:, 0]) & (v[1] < b[:, 4]) & (v[4] > b[:, 1])
+                 & (v[2] < b[:, 5]) & (v[5] > b[:, 2]) & (alive != k))
+            if m.any():
+                k2 = alive[np.argmax(m)]
+                w = t.a[k2]
+                t.a[k] = [min(v[0], w[0]), min(v[1], w[1]), min(v[2], w[2]),
+                          max(v[3], w[3]), max(v[4], w[4]), max(v[5], w[5])]
+                t.dead[k2] = True
+                changed = True
+    t.store()
 
 
 def _ratio(a: int, b: int) -> float:

@@ -19,8 +19,7 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .ccl import (finalize_particle_neighborhoods_, gen_particle_neighborhoods,
-                  update_particle_neighborhoods3d_, update_particle_neighborhoods_)
+from .ccl import link_slices
 
 __all__ = ["cu_dilate", "slice_bounding_rectangles", "particle_bounding_boxes", "particle_bounding_boxes_3d"]
 
@@ -89,25 +88,18 @@ def slice_bounding_rectangles(vol: torch.Tensor, threshold: Optional[float] = No
     return out
 
 
-def _bounding_boxes(vol, threshold, update):
-    per_slice = slice_bounding_rectangles(vol, threshold)
-    if not per_slice:
-        return {}
-    pbbs = gen_particle_neighborhoods(per_slice[0], 1)
-    for idx in range(2, len(per_slice) + 1):
-        update(pbbs, per_slice[idx - 1], idx)
-    finalize_particle_neighborhoods_(pbbs)
-    return pbbs
+def _bounding_boxes(vol, threshold, three_d):
+    return link_slices(slice_bounding_rectangles(vol, threshold), three_d)
 
 
 def particle_bounding_boxes(d_bin_vol: torch.Tensor, threshold: Optional[float] = None):
     """particle_detection.jl:58-77: neighbourhoods {UUID: [x_min, y_min, z_min, x_max, y_max,
     z_max]} of a Bool volume (or of ``d_vol .<= threshold`` for an N0f8 volume); rectangles
     overlapping in x-y are linked along z."""
-    return _bounding_boxes(d_bin_vol, threshold, update_particle_neighborhoods_)
+    return _bounding_boxes(d_bin_vol, threshold, False)
 
 
 def particle_bounding_boxes_3d(d_bin_vol: torch.Tensor, threshold: Optional[float] = None):
     """particle_detection.jl:79-96: as particle_bounding_boxes, linking a rectangle only to
     neighbourhoods that reach the previous slice."""
-    return _bounding_boxes(d_bin_vol, threshold, update_particle_neighborhoods3d_)
+    return _bounding_boxes(d_bin_vol, threshold, True)

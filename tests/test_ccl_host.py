@@ -58,3 +58,83 @@ def test_delete_duplicates_merges_transitively():
 
 def test_ratio_semantics():
     assert ccl._ratio(3, 0) == math.inf and math.isnan(ccl._ratio(0, 0))
+
+
+# scalar restatement of ccl.jl:314-466, the check for the vectorised implementation
+def _ov2(a, b):
+    return a[0] < b[2] and a[2] > b[0] and a[1] < b[3] and a[3] > b[1]
+
+
+def _scalar_update(pn, brs, s, three_d):
+    sub = [v for v in pn if v[5] == s - 1] if three_d else None
+    for br in brs:
+        hit = False
+        for v in (sub if three_d else list(pn)):
+            if _ov2(br, (v[0], v[1], v[3], v[4])):
+                v[0], v[1], v[3], v[4] = min(br[0], v[0]), min(br[1], v[1]), max(br[2], v[3]), max(br[3], v[4])
+                if three_d:
+                    v[5] = s
+                elif s < v[2]:
+                    v[2] = s
+                elif s > v[5]:
+                    v[5] = s
+                hit = True
+        if not hit:
+            pn.append([br[0], br[1], s, br[2], br[3], s])
+    pn[:] = [v for v in pn if not (abs(v[0] - v[3]) == 1 and abs(v[1] - v[4]) == 1 and abs(v[2] - v[5]) == 1)]
+
+
+def _scalar_dedup(pn):
+    changed = True
+    while changed:
+        changed = False
+        i = 0
+        while i < len(pn):
+            v = pn[i]
+            for j, w in enumerate(pn):
+                if j != i and ccl.judge_overlap3d(tuple(v), tuple(w)):
+                    v[:] = [min(v[0], w[0]), min(v[1], w[1]), min(v[2], w[2]),
+                            max(v[3], w[3]), max(v[4], w[4]), max(v[5], w[5])]
+                    del pn[j]
+                    if j < i:
+                        i -= 1
+                    changed = True
+                    break
+            i += 1
+
+
+import numpy as np  # noqa: E402
+import pytest  # noqa: E402
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_vectorised_linking_equals_scalar_loops(three_d, seed):
+    rng = np.random.default_rng(seed)
+    slices = [[tuple(int(v) for v in (x, y, x + w, y + h))
+               for x, y, w, h in zip(rng.integers(1, 60, k), rng.integers(1, 60, k), rng.integers(0, 8, k),
+                                     rng.integers(0, 8, k))] for k in rng.integers(0, 12, 15)]
+    pn = ccl.gen_particle_neighborhoods(slices[0], 1)
+    ref = [list(v) for v in pn.values()]
+    upd = ccl.update_particle_neighborhoods3d_ if three_d else ccl.update_particle_neighborhoods_
+    for z in range(2, len(slices) + 1):
+        upd(pn, slices[z - 1], z)
+        _scalar_update(ref, slices[z - 1], z, three_d)
+        assert list(pn.values()) == ref
+    ccl.delete_duplicates_(pn)
+    _scalar_dedup(ref)
+    assert list(pn.values()) == ref
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+def test_link_slices_equals_per_call_updates(three_d):
+    rng = np.random.default_rng(7)
+    slices = [[tuple(int(v) for v in (x, y, x + w, y + h))
+               for x, y, w, h in zip(rng.integers(1, 80, k), rng.integers(1, 80, k), rng.integers(2, 9, k),
+                                     rng.integers(2, 9, k))] for k in rng.integers(0, 10, 30)]
+    pn = ccl.gen_particle_neighborhoods(slices[0], 1)
+    upd = ccl.update_particle_neighborhoods3d_ if three_d else ccl.update_particle_neighborhoods_
+    for z in range(2, len(slices) + 1):
+        upd(pn, slices[z - 1], z)
+    ccl.finalize_particle_neighborhoods_(pn)
+    assert list(ccl.link_slices(slices, three_d).values()) == list(pn.values())
