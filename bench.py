@@ -248,8 +248,6 @@ def bench_detect(ph, torch, holo):
     headline hologram, thresholded at 30/255 inside the labelling kernel, 8-connected
     components and their rectangles for every slice (one libholo call; 1 B read per voxel plus
     the union-find state), then the host z-linking into neighbourhoods."""
-    import ctypes
-
     from particleholography import _lib
 
     d_sqr = ph.cu_transfer_sqrt_arr(N, LAM, DX)

@@ -773,7 +773,7 @@ int holo_create(holo_handle* out, int device) {
   if (cap > 0) h->max_lds = static_cast<int>(std::min<size_t>(cap, h->max_lds));
   h->col_e = static_cast<int>(env_size("HOLO_COL_E", 0));
   // Per-size tiles chosen by holo_autotune on an MI355X and confirmed on single-hologram
-  // reconstructions (768^2 and 1280x1024 +37%, 2048^2 +11%, 512^2 +5% over the generic rule);
+  // reconstructions (768^2 and 1280x1024 +37%, 2048^2 +14%, 512^2 +5% over the generic rule);
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
@@ -782,7 +782,7 @@ int holo_create(holo_handle* out, int device) {
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
                                        {768, 768, 56, 16, 4, 4, 2},
                                        {1280, 1024, 25, 8, 8, 4, 4},
-                                       {2048, 2048, 8, 8, 4, 4, 2}};
+                                       {2048, 2048, 16, 16, 4, 4, 2}};
     for (const Shipped& e : kShipped) {
       Tuning t;
       t.batch = e.batch;

@@ -11,6 +11,7 @@ using FixedPointNumbers
 export CuTransferSqrtPart, CuTransfer, CuWavefront, CuLowPassFilter
 export cu_rectangle_filter, cu_super_gaussian_filter, cu_apply_low_pass_filter, cu_apply_low_pass_filter!
 export cu_make_background_mode, cu_ingest_u8
+export cu_connected_component_labeling, cu_find_valid_labels, slice_bounding_rectangles, cu_dilate
 export cu_transfer_sqrt_arr, cu_transfer, cu_gabor_wavefront, cu_phase_retrieval_holo,
        cu_get_reconst_vol, cu_get_reconst_xyprojection, cu_get_reconst_vol_and_xyprojection,
        cu_get_reconst_complex_vol, cu_asm_prop!, cu_2d_pad, cu_get_reconst_vol_and_xyprojection_padded
@@ -267,6 +268,82 @@ function cu_ingest_u8(imgs::ROCArray{UInt8}, bkg::Union{Nothing,ROCArray{Float64
                  (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
                  _handle(), _stream(), nx, ny, count, pointer(imgs),
                  bkg === nothing ? C_NULL : pointer(bkg), offset, pointer(out)))
+    return out
+end
+
+# ---- ccl.jl:186-281 and the per-slice part of particle_detection.jl:58-96 ----------------------
+const HOLO_CCL_F32, HOLO_CCL_BINARY, HOLO_CCL_N0F8_LE = 0, 1, 2
+
+struct HoloBox            # holo_box
+    label::UInt32
+    i0::Int32
+    j0::Int32
+    i1::Int32
+    j1::Int32
+end
+
+_ccl_kind(::Type{Float32}) = HOLO_CCL_F32
+_ccl_kind(::Type{<:Union{Bool,UInt8,N0f8}}) = HOLO_CCL_BINARY
+
+# Labels (1 + the smallest linear index of the component, background 0), as the reference's
+# union-find leaves them.
+function cu_connected_component_labeling(input_img::ROCArray{T,2}) where {T}
+    @assert length(input_img) <= 2^32 - 1 "Image is too large. Maximum length is 2^32-1."
+    nx, ny = size(input_img)
+    out = ROCArray{UInt32}(undef, nx, ny)
+    _check(ccall((:holo_ccl_label, libholo), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Cint, Float64, Ptr{Cvoid}),
+                 _handle(), _stream(), nx, ny, 1, pointer(input_img), _ccl_kind(T), 0.0, pointer(out)))
+    return out
+end
+
+function _label_boxes(labels::ROCArray{UInt32,2}; max_boxes::Int=1 << 16)
+    nx, ny = size(labels)
+    while true
+        boxes = ROCArray{HoloBox}(undef, max_boxes)
+        nbox = ROCArray{Cint}(undef, 1)
+        _check(ccall((:holo_label_boxes, libholo), Cint,
+                     (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+                     _handle(), _stream(), nx, ny, 1, pointer(labels), max_boxes, pointer(boxes), pointer(nbox)))
+        n = Int(Array(nbox)[1])
+        n <= max_boxes && return sort(Array(boxes)[1:n]; by=b -> b.label)
+        max_boxes = n
+    end
+end
+
+cu_find_valid_labels(labels::ROCArray{UInt32,2}) = Int64[b.label for b in _label_boxes(labels)]
+
+# Rectangles (x_min, y_min, x_max, y_max) of the components of every slice, ascending label
+# order per slice: what particle_bounding_boxes gets from cu_connected_component_labeling +
+# cu_find_valid_labels + get_bounding_rectangles slice by slice, in one call without label
+# images on the host. `threshold` binarises an N0f8 volume as d_vol .<= threshold.
+function slice_bounding_rectangles(vol::ROCArray{T,3}; threshold=nothing, max_boxes::Int=4096) where {T}
+    nx, ny, slices = size(vol)
+    kind = threshold === nothing ? _ccl_kind(T) : HOLO_CCL_N0F8_LE
+    thr = threshold === nothing ? 0.0 : Float64(threshold)
+    while true
+        boxes = ROCArray{HoloBox}(undef, max_boxes * slices)
+        nbox = ROCArray{Cint}(undef, slices)
+        _check(ccall((:holo_ccl_boxes, libholo), Cint,
+                     (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Cint, Float64, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+                     _handle(), _stream(), nx, ny, slices, pointer(vol), kind, thr, max_boxes,
+                     pointer(boxes), pointer(nbox)))
+        n = Array(nbox)
+        if slices == 0 || maximum(n) <= max_boxes
+            b = Array(boxes)
+            return [[(Int(x.j0) + 1, Int(x.i0) + 1, Int(x.j1) + 1, Int(x.i1) + 1)
+                     for x in sort(b[(z - 1) * max_boxes + 1:(z - 1) * max_boxes + n[z]]; by=x -> x.label)]
+                    for z in 1:slices]
+        end
+        max_boxes = maximum(n)
+    end
+end
+
+function cu_dilate(vol::ROCArray{Bool,3}; blocksize=32)
+    nx, ny, slices = size(vol)
+    out = similar(vol)
+    _check(ccall((:holo_dilate, libholo), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+                 _handle(), _stream(), nx, ny, slices, pointer(vol), pointer(out)))
     return out
 end
 

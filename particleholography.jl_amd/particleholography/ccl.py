@@ -17,17 +17,16 @@ memory), so a rectangle's x comes from the tensor's row index and y from its col
 """
 from __future__ import annotations
 
-import ctypes
 import math
 import random
 import uuid
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Dict, List, Sequence, Tuple
 
 import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, holo_box
+from ._lib import check
 
 __all__ = ["cu_connected_component_labeling", "count_labels", "cu_find_valid_labels",
            "get_bounding_rectangles", "gen_particle_neighborhoods", "update_particle_neighborhoods_",

@@ -2,7 +2,6 @@
 groups on free slots, and every result equals the same request computed alone."""
 import numpy as np
 import pytest
-import torch
 
 pytestmark = pytest.mark.gpu
 
