@@ -67,6 +67,12 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3: one warm-up + --steps steps, then a 256 MiB device copy "
                          "(a kernel of known traffic to scale FETCH_SIZE / WRITE_SIZE); no extras")
+    ap.add_argument("--profile-detect", action="store_true",
+                    help="for rocprofv3: the particle-detection extra (1024^2 x 1000 N0f8 volume, "
+                         "labels and rectangles of every slice, --steps calls); no other work")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="for rocprofv3: the 64-hologram batched reconstruction (shipped tiles, "
+                         "--steps calls after one warm-up); no other work")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the z-range "
@@ -91,6 +97,18 @@ def main():
 
     def step():
         ph.reconst_xyprojection_holo(holo, h0, hdz, SLICES, out=proj)
+
+    if args.profile_detect:
+        r = bench_detect(ph, torch, holo, reps=args.steps)
+        log("bench: detection profile run done")
+        print(json.dumps({"profile_detect": True, "steps": args.steps, **r}), flush=True)
+        return
+    if args.profile_batch:
+        r = bench_batched(torch, dev, args.batch_holograms, autotune=False, reps=args.steps)
+        log("bench: batched profile run done")
+        print(json.dumps({"profile_batch": True, "steps": args.steps, "slices_1024eq_per_s": r["slices_1024eq_per_s"],
+                          "algorithmic_GBs": r["algorithmic_GBs"]}), flush=True)
+        return
 
     if args.profile_gs:
         i2 = torch.from_numpy(synth.gs_pair(103, 500, N, 2000.0)[1]).to(dev)
@@ -243,7 +261,7 @@ def bench_ingest(ph, torch, dev):
             "config": "1024^2 uint8 holograms: 64 ingested with a Float64 background, 256 reduced"}
 
 
-def bench_detect(ph, torch, holo):
+def bench_detect(ph, torch, holo, reps=5):
     """Particle detection downstream of reconstruction: the 1024^2 x 1000 N0f8 volume of the
     headline hologram, thresholded at 30/255 inside the labelling kernel, 8-connected
     components and their rectangles for every slice (one libholo call; 1 B read per voxel plus
@@ -263,7 +281,6 @@ def bench_detect(ph, torch, holo):
         _lib.check(lib.holo_ccl_boxes(h.ptr, st, N, N, SLICES, vol.data_ptr(), _lib.HOLO_CCL_N0F8_LE,
                                       30 / 255, max_boxes, boxes.data_ptr(), nbox.data_ptr()))
 
-    reps = 5
     _, sec = timed_steps(run, reps, 1, torch)
     t0 = time.perf_counter()
     pbbs = ph.particle_bounding_boxes(vol, threshold=30 / 255)
@@ -276,9 +293,10 @@ def bench_detect(ph, torch, holo):
             "config": "1024^2 x 1000 N0f8 volume, threshold 30/255, 8-connected, max 4096 boxes per slice"}
 
 
-def bench_batched(torch, dev, nholo):
+def bench_batched(torch, dev, nholo, autotune=True, reps=1):
     """64 seeded holograms, sides from {512, 768, 1024, 1280x1024, 2048}, 100-2000 disks,
-    128-1024 slices each over z 1e4-2e5 um; one batched call (tiles autotuned per size)."""
+    128-1024 slices each over z 1e4-2e5 um; one batched call (tiles autotuned per size unless
+    ``autotune`` is False: then the shipped tiles), timed over ``reps`` calls."""
     import numpy as np
 
     from particleholography import batch, synth
@@ -293,20 +311,22 @@ def bench_batched(torch, dev, nholo):
         h = synth.hologram_device(104 * 1000 + i, nd, nx, ny, zmin=1e4, zmax=2e5)
         jobs.append(batch.ReconJob(h, -1e4, -(2e5 - 1e4) / s, s, LAM, DX))
     before = {(nx, ny): batch.get_tuning_for_size(nx, ny) for nx, ny in SIZES}
-    tuned = {f"{nx}x{ny}": batch.autotune(nx, ny, LAM, DX, save=False) for nx, ny in SIZES}
+    tuned = {f"{nx}x{ny}": (batch.autotune(nx, ny, LAM, DX, save=False) if autotune else before[(nx, ny)])
+             for nx, ny in SIZES}
     batch.reconst_batch(jobs)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    batch.reconst_batch(jobs)
+    for _ in range(reps):
+        batch.reconst_batch(jobs)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt = (time.perf_counter() - t0) / reps
     slices = sum(j.slices for j in jobs)
     px_slices = sum(j.slices * j.holo.numel() for j in jobs)
     out = {"holograms": nholo, "slices": slices, "slices_per_s": round(slices / dt, 1),
            "slices_1024eq_per_s": round(px_slices / (N * N) / dt, 1),
            "algorithmic_GBs": round(32.0 * px_slices / dt / 1e9, 1), "wall_ms": round(dt * 1e3, 2),
            "tuning": tuned, "per_size": {}}
-    for nx, ny in SIZES:
+    for nx, ny in (SIZES if reps == 1 else []):
         sub = [j for j in jobs if tuple(j.holo.shape) == (ny, nx)]
         if not sub:
             continue
