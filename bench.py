@@ -195,6 +195,22 @@ def main():
         cpu = cpu_baseline.measure(holo_np, -Z0, -DZ, LAM, DX, SLICES, budget_s=15.0)
         cpu["value"] = round(cpu["value"], 2)
 
+    # per-hologram latency of the headline workload: submission to completion of one 1000-slice
+    # reconstruction, 20 calls each synchronised (the stream is idle at every submission)
+    lat = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    latency = {"hologram_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                               "p99": round(float(np.percentile(lat, 99)), 3), "n": len(lat),
+                               "workload": "one 1024^2 hologram, 1000 slices, idle stream"}}
+    if "serving" in extras:
+        srv = extras["serving"]["poisson_at_75pct"]
+        latency["served_ms"] = {"p50": srv["p50_ms"], "p99": srv["p99_ms"], "n": srv["requests"],
+                                "workload": "Poisson stream at 75% of capacity, mixed 512^2-2048^2, 20% GS"}
+
     out = {
         "metric": METRIC, "value": round(slices_per_s, 1), "unit": "z-slices/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -204,7 +220,7 @@ def main():
                                "at dz=200 um from z0=1e4 um, xy min-projection", "size": [N, N],
                    "slices": SLICES, "z0_um": Z0, "dz_um": DZ, "wavelength_um": LAM, "pitch_um": DX,
                    "transfer": "generated FP64-wrapped phase", "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "extras": extras,
+        "roofline": roofline, "latency": latency, "cpu_baseline": cpu, "extras": extras,
     }
     print(json.dumps(out), flush=True)
 

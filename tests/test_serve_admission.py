@@ -36,3 +36,19 @@ def test_oversized_request_runs_alone_when_idle():
     assert _ids(select_batch(w, 0, 100, 8, True)) == []        # others in flight: wait
     assert _ids(select_batch(w, 0, 100, 8, False)) == [0]      # idle: admitted alone
     assert _ids(select_batch(w, 0, 100, 8, False)) == [1]
+
+
+class _W(_R):
+    def __init__(self, rid, fp, work):
+        super().__init__(rid, fp)
+        self._work = work
+
+    def work_bytes(self):
+        return self._work
+
+
+def test_work_cap_closes_groups_but_admits_one():
+    w = deque([_W(0, 1, 30), _W(1, 1, 30), _W(2, 1, 50), _W(3, 1, 5)])
+    assert _ids(select_batch(w, 0, 1000, 8, False, max_work=64)) == [0, 1]   # 30 + 30 <= 64 < 110
+    assert _ids(select_batch(w, 0, 1000, 8, False, max_work=40)) == [2]      # first one always fits
+    assert _ids(select_batch(w, 0, 1000, 8, False, max_work=40)) == [3]
