@@ -138,77 +138,105 @@ __device__ __forceinline__ void tile_keys(const CclSrc& s, size_t base, int nx, 
   }
 }
 
-// Pass 1. grid (ceil(nx/64), ceil(ny/16), count). A tile without foreground only clears its
-// flag; otherwise the tile's pixels are unioned in LDS, P receives each foreground pixel's
-// tile-local root (background kNoLabel) and the flag is set.
+// Pass 1. grid (ceil(nx/64), ceil(ceil(ny/16)/kTileRows), count): a workgroup walks
+// kTileRows tiles down the image. A tile without foreground only clears its flag; otherwise
+// the tile's pixels are unioned in LDS, P receives each foreground pixel's tile-local root
+// (background kNoLabel), the flag is set and the tile is appended to `list` (global tile ids,
+// *count entries; arbitrary order). The workgroup at (0, 0, z) also zeroes nbox[z] when given.
+constexpr int kTileRows = 4;
 __global__ __launch_bounds__(256) void k_ccl_local(int nx, int ny, CclSrc src, unsigned* __restrict__ P,
-                                                   unsigned* __restrict__ flags) {
+                                                   unsigned* __restrict__ flags, unsigned* __restrict__ list,
+                                                   unsigned* __restrict__ count, int* __restrict__ nbox) {
   __shared__ unsigned key[kTilePx];
   __shared__ unsigned par[kTilePx];
   const int t = threadIdx.x;
   const int ly = t >> 4, lx0 = 4 * (t & 15);
   const int x0 = blockIdx.x * kTileX + lx0;
-  const int y0 = blockIdx.y * kTileY;
-  const int y = y0 + ly;
+  const int tiles_y = (ny + kTileY - 1) / kTileY;
   const size_t npx = static_cast<size_t>(nx) * ny;
   const size_t base = static_cast<size_t>(blockIdx.z) * npx;
-  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  unsigned k4[4];
-  tile_keys(src, base, nx, ny, x0, y, src.vec, k4);
   const int i0 = ly * kTileX + lx0;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    key[i0 + c] = k4[c];
-    par[i0 + c] = i0 + c;
-  }
-  const int any = __syncthreads_or((k4[0] | k4[1] | k4[2] | k4[3]) != 0u);
-  if (!any) {
-    if (t == 0) flags[tile] = 0u;
-    return;
-  }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int lx = lx0 + c, i = i0 + c;
-    const unsigned k = k4[c];
-    if (k == 0u) continue;
-    if (lx > 0 && key[i - 1] == k) lds_union(par, i, i - 1);
-    if (ly > 0) {
-      if (lx > 0 && key[i - kTileX - 1] == k) lds_union(par, i, i - kTileX - 1);
-      if (key[i - kTileX] == k) lds_union(par, i, i - kTileX);
-      if (lx < kTileX - 1 && key[i - kTileX + 1] == k) lds_union(par, i, i - kTileX + 1);
+  if (nbox && blockIdx.x == 0 && blockIdx.y == 0 && t == 0) nbox[blockIdx.z] = 0;
+  for (int r = 0; r < kTileRows; ++r) {
+    const int ty = blockIdx.y * kTileRows + r;
+    if (ty >= tiles_y) break;  // uniform across the workgroup
+    const int y0 = ty * kTileY;
+    const int y = y0 + ly;
+    const unsigned tile = (static_cast<unsigned>(blockIdx.z) * tiles_y + ty) * gridDim.x + blockIdx.x;
+    unsigned k4[4];
+    tile_keys(src, base, nx, ny, x0, y, src.vec, k4);
+    const int any = __syncthreads_or((k4[0] | k4[1] | k4[2] | k4[3]) != 0u);
+    if (!any) {
+      if (t == 0) flags[tile] = 0u;
+      continue;
     }
-  }
-  __syncthreads();
-  if (t == 0) flags[tile] = 1u;
-  if (y >= ny) return;
-  unsigned out[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    out[c] = kNoLabel;
-    if (k4[c] != 0u) {
-      const unsigned r = lds_find(par, i0 + c);
-      out[c] = static_cast<unsigned>((y0 + static_cast<int>(r) / kTileX) * nx + blockIdx.x * kTileX + r % kTileX);
+    for (int c = 0; c < 4; ++c) {
+      key[i0 + c] = k4[c];
+      par[i0 + c] = i0 + c;
     }
-  }
-  unsigned* dst = P + base + static_cast<size_t>(y) * nx + x0;
-  if (src.vec && x0 + 3 < nx) {
-    *reinterpret_cast<uint4*>(dst) = make_uint4(out[0], out[1], out[2], out[3]);
-  } else {
+    __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (x0 + c < nx) dst[c] = out[c];
+    for (int c = 0; c < 4; ++c) {
+      const int lx = lx0 + c, i = i0 + c;
+      const unsigned k = k4[c];
+      if (k == 0u) continue;
+      if (lx > 0 && key[i - 1] == k) lds_union(par, i, i - 1);
+      if (ly > 0) {
+        if (lx > 0 && key[i - kTileX - 1] == k) lds_union(par, i, i - kTileX - 1);
+        if (key[i - kTileX] == k) lds_union(par, i, i - kTileX);
+        if (lx < kTileX - 1 && key[i - kTileX + 1] == k) lds_union(par, i, i - kTileX + 1);
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      flags[tile] = 1u;
+      if (list) list[atomicAdd(count, 1u)] = tile;
+    }
+    if (y < ny) {
+      unsigned out[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        out[c] = kNoLabel;
+        if (k4[c] != 0u) {
+          const unsigned rt = lds_find(par, i0 + c);
+          out[c] = static_cast<unsigned>((y0 + static_cast<int>(rt) / kTileX) * nx + blockIdx.x * kTileX + rt % kTileX);
+        }
+      }
+      unsigned* dst = P + base + static_cast<size_t>(y) * nx + x0;
+      if (src.vec && x0 + 3 < nx) {
+        *reinterpret_cast<uint4*>(dst) = make_uint4(out[0], out[1], out[2], out[3]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (x0 + c < nx) dst[c] = out[c];
+      }
+    }
+    __syncthreads();  // the LDS of this tile is read before the next tile overwrites it
   }
 }
 
-// Pass 2. Same grid, 128 threads: t < 64 the top-row pixel lx = t (NW, N, NE across the top
-// edge), 64 <= t < 80 the left-column pixel ly = t - 64 (W, NW), 80 <= t < 96 the right-column
-// pixel ly = t - 80 (NE). Tiles without foreground have no edges to join.
-__global__ __launch_bounds__(128) void k_ccl_border(int nx, int ny, CclSrc src, unsigned* __restrict__ P,
-                                                    const unsigned* __restrict__ flags) {
-  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  if (flags[tile] == 0u) return;
+// A tile of the list: slice z, tile column tx, tile row ty.
+struct TileId {
+  int z, tx, ty;
+};
+__device__ __forceinline__ TileId tile_of(unsigned id, int nx, int ny) {
+  const int tiles_x = (nx + kTileX - 1) / kTileX, tiles_y = (ny + kTileY - 1) / kTileY;
+  TileId r;
+  r.tx = static_cast<int>(id % tiles_x);
+  id /= tiles_x;
+  r.ty = static_cast<int>(id % tiles_y);
+  r.z = static_cast<int>(id / tiles_y);
+  return r;
+}
+
+// Pass 2 on one tile, 128 threads: t < 64 the top-row pixel lx = t (NW, N, NE across the top
+// edge), 64 <= t < 80 the left-column pixel ly = t - 64 (W, NW), 80 <= t < 96 the
+// right-column pixel ly = t - 80 (NE).
+__device__ __forceinline__ void border_tile(int nx, int ny, const CclSrc& src, unsigned* __restrict__ P,
+                                            const TileId& tl) {
   const int t = threadIdx.x;
-  const int x0 = blockIdx.x * kTileX, y0 = blockIdx.y * kTileY;
+  const int x0 = tl.tx * kTileX, y0 = tl.ty * kTileY;
   int x, y;
   if (t < 64) {
     x = x0 + t;
@@ -224,7 +252,7 @@ __global__ __launch_bounds__(128) void k_ccl_border(int nx, int ny, CclSrc src, 
   }
   if (x >= nx || y >= ny) return;
   const size_t npx = static_cast<size_t>(nx) * ny;
-  const size_t base = static_cast<size_t>(blockIdx.z) * npx;
+  const size_t base = static_cast<size_t>(tl.z) * npx;
   unsigned* Ps = P + base;
   const unsigned me = static_cast<unsigned>(y) * nx + x;
   const unsigned c = ccl_key(src, base + me);
@@ -246,6 +274,15 @@ __global__ __launch_bounds__(128) void k_ccl_border(int nx, int ny, CclSrc src, 
   } else {
     link(x + 1, y - 1);
   }
+}
+
+// Pass 2 over the tiles of the list: a small grid walks it (tiles without foreground have no
+// edges to join and are not in it).
+__global__ __launch_bounds__(128) void k_ccl_border(int nx, int ny, CclSrc src, unsigned* __restrict__ P,
+                                                    const unsigned* __restrict__ list,
+                                                    const unsigned* __restrict__ count) {
+  const unsigned n = *count;
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) border_tile(nx, ny, src, P, tile_of(list[k], nx, ny));
 }
 
 // Pass 3 (labels), tile grid: labels = root + 1 for foreground pixels, 0 for background (a
@@ -300,6 +337,13 @@ __global__ void k_ccl_from_labels(size_t npx, int count, const uint32_t* __restr
   P[z * npx + idx] = valid ? l - 1u : kNoLabel;
 }
 
+// list[k] = k, *count = n: every tile listed (boxes of a given label image).
+__global__ void k_ccl_list_all(unsigned n, unsigned* __restrict__ list, unsigned* __restrict__ count) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) list[k] = k;
+  if (k == 0) *count = n;
+}
+
 // A box of one component of one slice: label = root offset + 1, coordinates 0-based
 // (i fast, j slow), inclusive.
 struct CclBox {
@@ -307,83 +351,89 @@ struct CclBox {
   int i0, j0, i1, j1;
 };
 
-// Pass 3a (boxes), tile grid: every foreground pixel points P at its root; a root claims a
-// box slot (slot order is arbitrary) and initialises it; M[root] = slot.
+// Pass 3a (boxes) over the list: every foreground pixel of a listed tile points P at its
+// root; a root claims a box slot (slot order is arbitrary) and initialises it; M[root] = slot.
 __global__ __launch_bounds__(256) void k_ccl_roots(int nx, int ny, unsigned* __restrict__ P,
-                                                   const unsigned* __restrict__ flags, unsigned* __restrict__ M,
+                                                   const unsigned* __restrict__ list,
+                                                   const unsigned* __restrict__ count, unsigned* __restrict__ M,
                                                    CclBox* __restrict__ boxes, int max_boxes, int* __restrict__ nbox) {
-  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  if (flags[tile] == 0u) return;
+  const unsigned n = *count;
   const int t = threadIdx.x;
-  const int x0 = blockIdx.x * kTileX + 4 * (t & 15);
-  const int y = blockIdx.y * kTileY + (t >> 4);
-  if (y >= ny) return;
-  const size_t z = blockIdx.z;
   const size_t npx = static_cast<size_t>(nx) * ny;
-  unsigned* Ps = P + z * npx;
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+    const TileId tl = tile_of(list[k], nx, ny);
+    const int x0 = tl.tx * kTileX + 4 * (t & 15);
+    const int y = tl.ty * kTileY + (t >> 4);
+    if (y >= ny) continue;
+    const size_t z = tl.z;
+    unsigned* Ps = P + z * npx;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    if (x0 + c >= nx) continue;
-    const unsigned idx = static_cast<unsigned>(y) * nx + x0 + c;
-    const unsigned p = Ps[idx];
-    if (p == kNoLabel) continue;
-    const unsigned r = g_find(Ps, p);
-    if (r == idx) {
-      const int slot = atomicAdd(nbox + z, 1);
-      M[z * npx + idx] = static_cast<unsigned>(slot);
-      if (slot < max_boxes) {
-        CclBox b;
-        b.label = r + 1u;
-        b.i0 = 0x7fffffff;
-        b.j0 = 0x7fffffff;
-        b.i1 = -1;
-        b.j1 = -1;
-        boxes[z * max_boxes + slot] = b;
+    for (int c = 0; c < 4; ++c) {
+      if (x0 + c >= nx) continue;
+      const unsigned idx = static_cast<unsigned>(y) * nx + x0 + c;
+      const unsigned p = Ps[idx];
+      if (p == kNoLabel) continue;
+      const unsigned r = g_find(Ps, p);
+      if (r == idx) {
+        const int slot = atomicAdd(nbox + z, 1);
+        M[z * npx + idx] = static_cast<unsigned>(slot);
+        if (slot < max_boxes) {
+          CclBox b;
+          b.label = r + 1u;
+          b.i0 = 0x7fffffff;
+          b.j0 = 0x7fffffff;
+          b.i1 = -1;
+          b.j1 = -1;
+          boxes[z * max_boxes + slot] = b;
+        }
       }
+      Ps[idx] = r;  // every foreground entry now points at its root
     }
-    Ps[idx] = r;  // every foreground entry now points at its root
   }
 }
 
-// Pass 3b (boxes), tile grid: a thread's 4 pixels issue one set of min / max atomics per run
-// of equal roots.
+// Pass 3b (boxes) over the list: a thread's 4 pixels issue one set of min / max atomics per
+// run of equal roots.
 __global__ __launch_bounds__(256) void k_ccl_boxes(int nx, int ny, const unsigned* __restrict__ P,
-                                                   const unsigned* __restrict__ flags,
+                                                   const unsigned* __restrict__ list,
+                                                   const unsigned* __restrict__ count,
                                                    const unsigned* __restrict__ M, CclBox* __restrict__ boxes,
                                                    int max_boxes) {
-  const size_t tile = (static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  if (flags[tile] == 0u) return;
+  const unsigned n = *count;
   const int t = threadIdx.x;
-  const int x0 = blockIdx.x * kTileX + 4 * (t & 15);
-  const int j = blockIdx.y * kTileY + (t >> 4);
-  if (j >= ny) return;
-  const size_t z = blockIdx.z;
   const size_t npx = static_cast<size_t>(nx) * ny;
-  const unsigned* Ps = P + z * npx + static_cast<size_t>(j) * nx;
-  const unsigned* Ms = M + z * npx;
-  CclBox* bz = boxes + z * max_boxes;
-  unsigned cur = kNoLabel;
-  int first = 0, last = 0;
-  auto flush = [&]() {
-    if (cur == kNoLabel) return;
-    const unsigned s = Ms[cur];
-    if (s >= static_cast<unsigned>(max_boxes)) return;
-    atomicMin(&bz[s].i0, first);
-    atomicMax(&bz[s].i1, last);
-    atomicMin(&bz[s].j0, j);
-    atomicMax(&bz[s].j1, j);
-  };
-  const int i_end = min(x0 + 4, nx);
-  for (int i = x0; i < i_end; ++i) {
-    const unsigned r = Ps[i];
-    if (r != cur) {
-      flush();
-      cur = r;
-      first = i;
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+    const TileId tl = tile_of(list[k], nx, ny);
+    const int x0 = tl.tx * kTileX + 4 * (t & 15);
+    const int j = tl.ty * kTileY + (t >> 4);
+    if (j >= ny) continue;
+    const size_t z = tl.z;
+    const unsigned* Ps = P + z * npx + static_cast<size_t>(j) * nx;
+    const unsigned* Ms = M + z * npx;
+    CclBox* bz = boxes + z * max_boxes;
+    unsigned cur = kNoLabel;
+    int first = 0, last = 0;
+    auto flush = [&]() {
+      if (cur == kNoLabel) return;
+      const unsigned s = Ms[cur];
+      if (s >= static_cast<unsigned>(max_boxes)) return;
+      atomicMin(&bz[s].i0, first);
+      atomicMax(&bz[s].i1, last);
+      atomicMin(&bz[s].j0, j);
+      atomicMax(&bz[s].j1, j);
+    };
+    const int i_end = min(x0 + 4, nx);
+    for (int i = x0; i < i_end; ++i) {
+      const unsigned r = Ps[i];
+      if (r != cur) {
+        flush();
+        cur = r;
+        first = i;
+      }
+      last = i;
     }
-    last = i;
+    flush();
   }
-  flush();
 }
 
 // cu_dilate (particle_detection.jl:12-42): out = OR of the 3 x 3 in-plane neighbourhood for

@@ -1052,10 +1052,11 @@ CclGrid ccl_grid(int nx, int ny, int c) {
 }
 
 // Slices per round so that the union-find state (P and the root slots M, 8 B/px, plus the
-// tile flags) stays within 256 MiB; at least one.
+// tile flags and list) stays within 1 GiB; at least one. P and M are written only in tiles
+// with foreground, so a larger round costs address space, not traffic, and saves launches.
 int ccl_round(size_t npx, int count) {
   const size_t per = 8 * npx + npx / 64;
-  size_t r = per ? (size_t(256) << 20) / per : 1;
+  size_t r = per ? env_size("HOLO_CCL_ROUND_BYTES", size_t(1) << 30) / per : 1;
   if (r < 1) r = 1;
   if (r > 65535) r = 65535;
   return static_cast<int>(std::min<size_t>(r, static_cast<size_t>(count)));
@@ -1065,35 +1066,42 @@ struct CclWs {
   unsigned* P;
   unsigned* M;
   unsigned* flags;
+  unsigned* list;
+  unsigned* count;
 };
 
 int ccl_workspace(holo_context* h, int nx, int ny, int round, bool with_m, CclWs* ws) {
   const size_t npx = static_cast<size_t>(nx) * ny;
   const size_t tiles = ccl_grid(nx, ny, 1).per_slice * round;
-  const size_t words = static_cast<size_t>(round) * npx * (with_m ? 2 : 1) + tiles;
+  const size_t pm = static_cast<size_t>(round) * npx * (with_m ? 2 : 1);
+  const size_t words = pm + 2 * tiles + 1;
   HOLO_TRY(ensure_ws(h, words * 4));
   ws->P = static_cast<unsigned*>(h->ws);
   ws->M = with_m ? ws->P + static_cast<size_t>(round) * npx : nullptr;
-  ws->flags = ws->P + static_cast<size_t>(round) * npx * (with_m ? 2 : 1);
+  ws->flags = ws->P + pm;
+  ws->list = ws->flags + tiles;
+  ws->count = ws->list + tiles;
   return HOLO_OK;
 }
 
-int ccl_passes_1_2(int nx, int ny, int c, const CclSrc& src, const CclWs& ws, hipStream_t st) {
+// Workgroups of the passes that walk the tile list (tiles with foreground only).
+constexpr unsigned kListGrid = 1024;
+
+int ccl_passes_1_2(int nx, int ny, int c, const CclSrc& src, const CclWs& ws, int* nbox, hipStream_t st) {
   const CclGrid g = ccl_grid(nx, ny, c);
-  hipLaunchKernelGGL(k_ccl_local, g.tiles, dim3(256), 0, st, nx, ny, src, ws.P, ws.flags);
+  HOLO_TRY(launch_fill(reinterpret_cast<float*>(ws.count), 1, 0u, st));
+  const dim3 lgrid(g.tiles.x, (g.tiles.y + kTileRows - 1) / kTileRows, c);
+  hipLaunchKernelGGL(k_ccl_local, lgrid, dim3(256), 0, st, nx, ny, src, ws.P, ws.flags, ws.list, ws.count, nbox);
   HOLO_TRY(check_launch());
-  hipLaunchKernelGGL(k_ccl_border, g.tiles, dim3(128), 0, st, nx, ny, src, ws.P, ws.flags);
+  hipLaunchKernelGGL(k_ccl_border, dim3(kListGrid), dim3(128), 0, st, nx, ny, src, ws.P, ws.list, ws.count);
   return check_launch();
 }
 
-int ccl_boxes_round(int nx, int ny, int c, const CclWs& ws, holo_box* boxes, int max_boxes, int* nbox,
-                    hipStream_t st) {
-  const CclGrid g = ccl_grid(nx, ny, c);
-  HOLO_TRY(launch_fill(reinterpret_cast<float*>(nbox), c, 0u, st));
-  hipLaunchKernelGGL(k_ccl_roots, g.tiles, dim3(256), 0, st, nx, ny, ws.P, ws.flags, ws.M,
+int ccl_boxes_round(int nx, int ny, const CclWs& ws, holo_box* boxes, int max_boxes, int* nbox, hipStream_t st) {
+  hipLaunchKernelGGL(k_ccl_roots, dim3(kListGrid), dim3(256), 0, st, nx, ny, ws.P, ws.list, ws.count, ws.M,
                      reinterpret_cast<CclBox*>(boxes), max_boxes, nbox);
   HOLO_TRY(check_launch());
-  hipLaunchKernelGGL(k_ccl_boxes, g.tiles, dim3(256), 0, st, nx, ny, ws.P, ws.flags, ws.M,
+  hipLaunchKernelGGL(k_ccl_boxes, dim3(kListGrid), dim3(256), 0, st, nx, ny, ws.P, ws.list, ws.count, ws.M,
                      reinterpret_cast<CclBox*>(boxes), max_boxes);
   return check_launch();
 }
@@ -1125,7 +1133,7 @@ int holo_ccl_label(holo_handle h, void* stream, int nx, int ny, int count, const
     const int c = std::min(round, count - z0);
     CclSrc s = src;
     s.img = static_cast<const uint8_t*>(img) + static_cast<size_t>(z0) * npx * esz;
-    HOLO_TRY(ccl_passes_1_2(nx, ny, c, s, ws, st));
+    HOLO_TRY(ccl_passes_1_2(nx, ny, c, s, ws, nullptr, st));
     hipLaunchKernelGGL(k_ccl_labels, ccl_grid(nx, ny, c).tiles, dim3(256), 0, st, nx, ny, ws.P, ws.flags,
                        labels + static_cast<size_t>(z0) * npx, lvec);
     HOLO_TRY(check_launch());
@@ -1150,9 +1158,8 @@ int holo_ccl_boxes(holo_handle h, void* stream, int nx, int ny, int count, const
     const int c = std::min(round, count - z0);
     CclSrc s = src;
     s.img = static_cast<const uint8_t*>(img) + static_cast<size_t>(z0) * npx * esz;
-    HOLO_TRY(ccl_passes_1_2(nx, ny, c, s, ws, st));
-    HOLO_TRY(ccl_boxes_round(nx, ny, c, ws, boxes + static_cast<size_t>(z0) * max_boxes, max_boxes,
-                             nboxes + z0, st));
+    HOLO_TRY(ccl_passes_1_2(nx, ny, c, s, ws, nboxes + z0, st));
+    HOLO_TRY(ccl_boxes_round(nx, ny, ws, boxes + static_cast<size_t>(z0) * max_boxes, max_boxes, nboxes + z0, st));
   }
   return HOLO_OK;
 }
@@ -1172,9 +1179,11 @@ int holo_label_boxes(holo_handle h, void* stream, int nx, int ny, int count, con
     hipLaunchKernelGGL(k_ccl_from_labels, dim3(static_cast<unsigned>((npx + 255) / 256), c), dim3(256), 0, st,
                        npx, c, labels + static_cast<size_t>(z0) * npx, ws.P);
     HOLO_TRY(check_launch());
-    HOLO_TRY(launch_fill(reinterpret_cast<float*>(ws.flags), ccl_grid(nx, ny, c).per_slice * c, 1u, st));
-    HOLO_TRY(ccl_boxes_round(nx, ny, c, ws, boxes + static_cast<size_t>(z0) * max_boxes, max_boxes,
-                             nboxes + z0, st));
+    const unsigned ntiles = static_cast<unsigned>(ccl_grid(nx, ny, c).per_slice * c);
+    hipLaunchKernelGGL(k_ccl_list_all, dim3((ntiles + 255) / 256), dim3(256), 0, st, ntiles, ws.list, ws.count);
+    HOLO_TRY(check_launch());
+    HOLO_TRY(launch_fill(reinterpret_cast<float*>(nboxes + z0), c, 0u, st));
+    HOLO_TRY(ccl_boxes_round(nx, ny, ws, boxes + static_cast<size_t>(z0) * max_boxes, max_boxes, nboxes + z0, st));
   }
   return HOLO_OK;
 }
