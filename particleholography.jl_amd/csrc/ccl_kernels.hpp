@@ -7,13 +7,13 @@
 // pixels are 0: the labels the reference's union-find produces (every union links the
 // larger root under the smaller, so the surviving root is the first pixel in memory order).
 //
-// Three passes over a slice, all on a grid of 64 x 16 tiles:
-//   1. k_ccl_local: a workgroup unions its tile's pixels in LDS and writes for every
+// The passes over a slice, on a grid of 64 x 16 tiles:
+//   0. k_ccl_scan: one wave per tile reads it with 16-byte loads and votes; occupied tiles are
+//      flagged and listed, so a sparse volume costs one streaming read of the image;
+//   1. k_ccl_local: a workgroup per listed tile unions its pixels in LDS and writes for every
 //      foreground pixel the global offset of its tile-local root (P), background 0xffffffff;
-//      a tile without foreground writes only its flag (0), and later passes skip it, so a
-//      sparse volume costs one read of the image;
-//   2. k_ccl_border: unions across tile edges only (W and NW of the left column, NW/N/NE of
-//      the top row, NE of the right column), lock-free on P with atomicMin;
+//   2. k_ccl_border: unions across the edges of listed tiles only (W and NW of the left
+//      column, NW/N/NE of the top row, NE of the right column), lock-free on P with atomicMin;
 //   3. k_ccl_labels (labels = root + 1), or k_ccl_roots + k_ccl_boxes (one box slot per root,
 //      then min / max of the coordinates of every run of equal roots along a row).
 #pragma once
@@ -138,81 +138,66 @@ __device__ __forceinline__ void tile_keys(const CclSrc& s, size_t base, int nx, 
   }
 }
 
-// Pass 1. grid (ceil(nx/64), ceil(ceil(ny/16)/kTileRows), count): a workgroup walks
-// kTileRows tiles down the image. A tile without foreground only clears its flag; otherwise
-// the tile's pixels are unioned in LDS, P receives each foreground pixel's tile-local root
-// (background kNoLabel), the flag is set and the tile is appended to `list` (global tile ids,
-// *count entries; arbitrary order). The workgroup at (0, 0, z) also zeroes nbox[z] when given.
-constexpr int kTileRows = 4;
-__global__ __launch_bounds__(256) void k_ccl_local(int nx, int ny, CclSrc src, unsigned* __restrict__ P,
-                                                   unsigned* __restrict__ flags, unsigned* __restrict__ list,
-                                                   unsigned* __restrict__ count, int* __restrict__ nbox) {
-  __shared__ unsigned key[kTilePx];
-  __shared__ unsigned par[kTilePx];
-  const int t = threadIdx.x;
-  const int ly = t >> 4, lx0 = 4 * (t & 15);
-  const int x0 = blockIdx.x * kTileX + lx0;
-  const int tiles_y = (ny + kTileY - 1) / kTileY;
+// Occupancy scan: one wave per tile, the whole tile read with 16-byte loads (u8: 64 lanes x
+// 16 B = the tile's 16 rows x 64 pixels; F32: 4 such sweeps), a wave vote, then lane 0 sets the
+// tile's flag and appends an occupied tile to `list`. No LDS and no barrier, so the pass
+// streams the volume. `vec16`: nx % 16 == 0 and the image 16-byte aligned (else byte/float
+// loads). grid-stride over all tiles of the round.
+__device__ __forceinline__ bool key_any16(const CclSrc& s, const uint4 w) {
+  if (s.kind == CCL_F32) {
+    return __uint_as_float(w.x) > 0.0f || __uint_as_float(w.y) > 0.0f || __uint_as_float(w.z) > 0.0f ||
+           __uint_as_float(w.w) > 0.0f;
+  }
+  const unsigned words[4] = {w.x, w.y, w.z, w.w};
+  bool a = false;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const unsigned v = (words[q] >> (8 * b)) & 0xffu;
+      a |= s.kind == CCL_BINARY ? v != 0u : v < s.lim;
+    }
+  return a;
+}
+
+__global__ __launch_bounds__(256) void k_ccl_scan(int nx, int ny, int count, CclSrc src, int vec16,
+                                                  unsigned* __restrict__ flags, unsigned* __restrict__ list,
+                                                  unsigned* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int tiles_x = (nx + kTileX - 1) / kTileX, tiles_y = (ny + kTileY - 1) / kTileY;
+  const unsigned ntiles = static_cast<unsigned>(tiles_x) * tiles_y * count;
   const size_t npx = static_cast<size_t>(nx) * ny;
-  const size_t base = static_cast<size_t>(blockIdx.z) * npx;
-  const int i0 = ly * kTileX + lx0;
-  if (nbox && blockIdx.x == 0 && blockIdx.y == 0 && t == 0) nbox[blockIdx.z] = 0;
-  for (int r = 0; r < kTileRows; ++r) {
-    const int ty = blockIdx.y * kTileRows + r;
-    if (ty >= tiles_y) break;  // uniform across the workgroup
-    const int y0 = ty * kTileY;
-    const int y = y0 + ly;
-    const unsigned tile = (static_cast<unsigned>(blockIdx.z) * tiles_y + ty) * gridDim.x + blockIdx.x;
-    unsigned k4[4];
-    tile_keys(src, base, nx, ny, x0, y, src.vec, k4);
-    const int any = __syncthreads_or((k4[0] | k4[1] | k4[2] | k4[3]) != 0u);
-    if (!any) {
-      if (t == 0) flags[tile] = 0u;
-      continue;
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      key[i0 + c] = k4[c];
-      par[i0 + c] = i0 + c;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int lx = lx0 + c, i = i0 + c;
-      const unsigned k = k4[c];
-      if (k == 0u) continue;
-      if (lx > 0 && key[i - 1] == k) lds_union(par, i, i - 1);
-      if (ly > 0) {
-        if (lx > 0 && key[i - kTileX - 1] == k) lds_union(par, i, i - kTileX - 1);
-        if (key[i - kTileX] == k) lds_union(par, i, i - kTileX);
-        if (lx < kTileX - 1 && key[i - kTileX + 1] == k) lds_union(par, i, i - kTileX + 1);
-      }
-    }
-    __syncthreads();
-    if (t == 0) {
-      flags[tile] = 1u;
-      if (list) list[atomicAdd(count, 1u)] = tile;
-    }
-    if (y < ny) {
-      unsigned out[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        out[c] = kNoLabel;
-        if (k4[c] != 0u) {
-          const unsigned rt = lds_find(par, i0 + c);
-          out[c] = static_cast<unsigned>((y0 + static_cast<int>(rt) / kTileX) * nx + blockIdx.x * kTileX + rt % kTileX);
-        }
-      }
-      unsigned* dst = P + base + static_cast<size_t>(y) * nx + x0;
-      if (src.vec && x0 + 3 < nx) {
-        *reinterpret_cast<uint4*>(dst) = make_uint4(out[0], out[1], out[2], out[3]);
+  const int esz = src.kind == CCL_F32 ? 4 : 1;
+  const int sweeps = esz;          // 16-byte lanes: 16 pixels (u8) or 4 pixels (f32) each
+  const int lanes_per_row = 4 * esz;  // lanes covering the 64 pixels of a row
+  const int rows_per_sweep = 64 / lanes_per_row;
+  for (unsigned tid = wave; tid < ntiles; tid += nwaves) {
+    const int tx = static_cast<int>(tid % tiles_x);
+    const int ty = static_cast<int>((tid / tiles_x) % tiles_y);
+    const size_t z = tid / (static_cast<unsigned>(tiles_x) * tiles_y);
+    const size_t base = z * npx;
+    bool any = false;
+    for (int sw = 0; sw < sweeps; ++sw) {
+      const int row = sw * rows_per_sweep + lane / lanes_per_row;
+      const int y = ty * kTileY + row;
+      const int px0 = (lane % lanes_per_row) * (16 / esz);
+      const int x = tx * kTileX + px0;
+      if (y >= ny || x >= nx) continue;
+      const size_t o = base + static_cast<size_t>(y) * nx + x;
+      if (vec16 && x + 16 / esz <= nx) {
+        const uint4 w = *reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(src.img) + o * esz);
+        any |= key_any16(src, w);
       } else {
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-          if (x0 + c < nx) dst[c] = out[c];
+        for (int c = 0; c < 16 / esz && x + c < nx; ++c) any |= ccl_key(src, o + c) != 0u;
       }
     }
-    __syncthreads();  // the LDS of this tile is read before the next tile overwrites it
+    const bool tile_any = __any(any);
+    if (lane == 0) {
+      flags[tid] = tile_any ? 1u : 0u;
+      if (tile_any) list[atomicAdd(cnt, 1u)] = tid;
+    }
   }
 }
 
@@ -228,6 +213,69 @@ __device__ __forceinline__ TileId tile_of(unsigned id, int nx, int ny) {
   r.ty = static_cast<int>(id % tiles_y);
   r.z = static_cast<int>(id / tiles_y);
   return r;
+}
+
+// Pass 1 over the occupied tiles of the list (from k_ccl_scan): the tile's pixels are unioned
+// in LDS and P receives each foreground pixel's tile-local root (background kNoLabel).
+// grid-stride over the list.
+__global__ __launch_bounds__(256) void k_ccl_local(int nx, int ny, CclSrc src, unsigned* __restrict__ P,
+                                                   const unsigned* __restrict__ list,
+                                                   const unsigned* __restrict__ cnt) {
+  __shared__ unsigned key[kTilePx];
+  __shared__ unsigned par[kTilePx];
+  const unsigned n = *cnt;
+  const int t = threadIdx.x;
+  const int ly = t >> 4, lx0 = 4 * (t & 15);
+  const size_t npx = static_cast<size_t>(nx) * ny;
+  const int i0 = ly * kTileX + lx0;
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+    const TileId tl = tile_of(list[k], nx, ny);
+    const int x0 = tl.tx * kTileX + lx0;
+    const int y0 = tl.ty * kTileY;
+    const int y = y0 + ly;
+    const size_t base = static_cast<size_t>(tl.z) * npx;
+    unsigned k4[4];
+    tile_keys(src, base, nx, ny, x0, y, src.vec, k4);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      key[i0 + c] = k4[c];
+      par[i0 + c] = i0 + c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int lx = lx0 + c, i = i0 + c;
+      const unsigned kk = k4[c];
+      if (kk == 0u) continue;
+      if (lx > 0 && key[i - 1] == kk) lds_union(par, i, i - 1);
+      if (ly > 0) {
+        if (lx > 0 && key[i - kTileX - 1] == kk) lds_union(par, i, i - kTileX - 1);
+        if (key[i - kTileX] == kk) lds_union(par, i, i - kTileX);
+        if (lx < kTileX - 1 && key[i - kTileX + 1] == kk) lds_union(par, i, i - kTileX + 1);
+      }
+    }
+    __syncthreads();
+    if (y < ny) {
+      unsigned out[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        out[c] = kNoLabel;
+        if (k4[c] != 0u) {
+          const unsigned rt = lds_find(par, i0 + c);
+          out[c] = static_cast<unsigned>((y0 + static_cast<int>(rt) / kTileX) * nx + tl.tx * kTileX + rt % kTileX);
+        }
+      }
+      unsigned* dst = P + base + static_cast<size_t>(y) * nx + x0;
+      if (src.vec && x0 + 3 < nx) {
+        *reinterpret_cast<uint4*>(dst) = make_uint4(out[0], out[1], out[2], out[3]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (x0 + c < nx) dst[c] = out[c];
+      }
+    }
+    __syncthreads();  // the LDS of this tile is read before the next tile overwrites it
+  }
 }
 
 // Pass 2 on one tile, 128 threads: t < 64 the top-row pixel lx = t (NW, N, NE across the top

@@ -1087,11 +1087,18 @@ int ccl_workspace(holo_context* h, int nx, int ny, int round, bool with_m, CclWs
 // Workgroups of the passes that walk the tile list (tiles with foreground only).
 constexpr unsigned kListGrid = 1024;
 
+// Scan (flags + list of occupied tiles), tile-local unions, cross-tile unions for one round
+// of c slices. nbox (boxes path) is zeroed here.
 int ccl_passes_1_2(int nx, int ny, int c, const CclSrc& src, const CclWs& ws, int* nbox, hipStream_t st) {
   const CclGrid g = ccl_grid(nx, ny, c);
   HOLO_TRY(launch_fill(reinterpret_cast<float*>(ws.count), 1, 0u, st));
-  const dim3 lgrid(g.tiles.x, (g.tiles.y + kTileRows - 1) / kTileRows, c);
-  hipLaunchKernelGGL(k_ccl_local, lgrid, dim3(256), 0, st, nx, ny, src, ws.P, ws.flags, ws.list, ws.count, nbox);
+  if (nbox) HOLO_TRY(launch_fill(reinterpret_cast<float*>(nbox), c, 0u, st));
+  const int vec16 = (nx % 16 == 0) && (reinterpret_cast<uintptr_t>(src.img) % 16 == 0);
+  const size_t ntiles = g.per_slice * c;
+  const unsigned sgrid = static_cast<unsigned>(std::min<size_t>(4096, (ntiles + 3) / 4));
+  hipLaunchKernelGGL(k_ccl_scan, dim3(sgrid), dim3(256), 0, st, nx, ny, c, src, vec16, ws.flags, ws.list, ws.count);
+  HOLO_TRY(check_launch());
+  hipLaunchKernelGGL(k_ccl_local, dim3(kListGrid), dim3(256), 0, st, nx, ny, src, ws.P, ws.list, ws.count);
   HOLO_TRY(check_launch());
   hipLaunchKernelGGL(k_ccl_border, dim3(kListGrid), dim3(128), 0, st, nx, ny, src, ws.P, ws.list, ws.count);
   return check_launch();
