@@ -53,6 +53,7 @@ struct Plan1D {
 
 struct Tuning {
   int batch = 0, spg_cols = 0, spg_rows = 0, cols = 0, rows = 0;
+  int erow = 0, ecol = 0;  // elements per thread of row / column passes (0: the length's default)
 };
 
 // E = elements per thread; the set below is what the kernels are instantiated for.
@@ -278,18 +279,28 @@ Tuning eff_tuning(const holo_context* h, int nx, int ny) {
   return t;
 }
 
-// Tiles of the Gerchberg-Saxton passes. A single pair is one plane per launch, so its tiles
-// trade width for workgroup count differently from reconstruction: measured on MI355X,
-// 4 columns x 1 row at 1024^2 (+6% iterations/s) and 4 x 2 at 512^2 (+13%); other sizes keep
-// the generic rule. Global tuning (holo_set_tuning) still overrides.
+// Tiles of the Gerchberg-Saxton passes. A single pair is one plane per launch: at the
+// reconstruction's 16 elements per thread a 1024^2 row pass is one wave per SIMD, so GS runs
+// its lines with fewer elements per thread (more waves, one more LDS exchange) and narrower
+// tiles. Measured on MI355X (iterations/s over the generic choice): 1024^2 E 8/8, 4 columns,
+// 1 row +15%; 512^2 E 4/8, 4 x 1 +8%; 2048^2 E 8 rows, 4 x 1 +5%. Global tuning
+// (holo_set_tuning) still sets the tile sizes.
 Tuning gs_tiles(const holo_context* h, int nx, int ny) {
   Tuning t;
   if (nx == 1024 && ny == 1024) {
+    t.erow = 8;
+    t.ecol = 8;
     t.cols = 4;
     t.rows = 1;
   } else if (nx == 512 && ny == 512) {
+    t.erow = 4;
+    t.ecol = 8;
     t.cols = 4;
-    t.rows = 2;
+    t.rows = 1;
+  } else if (nx == 2048 && ny == 2048) {
+    t.erow = 8;
+    t.cols = 4;
+    t.rows = 1;
   }
   if (h->tune.cols > 0) t.cols = h->tune.cols;
   if (h->tune.rows > 0) t.rows = h->tune.rows;
@@ -302,12 +313,13 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
               const Tuning* tiles = nullptr) {
   if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
   const Plan1D *px, *py;
-  HOLO_TRY(get_plan(h, nx, &px));
-  // column passes may run with fewer elements per thread (more threads per column, fewer
-  // registers) when the length allows it: E must divide L, one column must fit a workgroup
-  int ecol = 0;
-  if (h->col_e > 0 && ny % h->col_e == 0 && ny / h->col_e <= 512 && h->col_e < choose_E(ny)) ecol = h->col_e;
-  HOLO_TRY(get_plan(h, ny, &py, ecol));
+  // passes may run with fewer elements per thread (more threads per line, fewer registers)
+  // when the length allows it: E must divide L and a line must fit a workgroup
+  auto fewer = [](int want, int L) {
+    return (want > 0 && L % want == 0 && L / want <= 512 && want < choose_E(L)) ? want : 0;
+  };
+  HOLO_TRY(get_plan(h, nx, &px, fewer(tiles ? tiles->erow : 0, nx)));
+  HOLO_TRY(get_plan(h, ny, &py, fewer(tiles && tiles->ecol ? tiles->ecol : h->col_e, ny)));
   std::memset(g, 0, sizeof(Geom));
   g->nx = nx;
   g->ny = ny;
