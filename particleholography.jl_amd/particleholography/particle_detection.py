@@ -12,7 +12,7 @@ Volumes are tensors of shape (slices, ny, nx): the reference's (nx, ny, slices) 
 """
 from __future__ import annotations
 
-from typing import List, Optional, Tuple
+from typing import Callable, Dict, List, Optional, Tuple
 
 import numpy as np
 import torch
@@ -21,7 +21,8 @@ from . import _lib
 from ._lib import check
 from .ccl import link_slices
 
-__all__ = ["cu_dilate", "slice_bounding_rectangles", "particle_bounding_boxes", "particle_bounding_boxes_3d"]
+__all__ = ["cu_dilate", "slice_bounding_rectangles", "particle_bounding_boxes", "particle_bounding_boxes_3d",
+           "tamura", "depth_profile", "getcenterfromslice", "gaussian_kernel", "particle_coordinates"]
 
 
 def _dev(t: torch.Tensor) -> int:
@@ -103,3 +104,64 @@ def particle_bounding_boxes_3d(d_bin_vol: torch.Tensor, threshold: Optional[floa
     """particle_detection.jl:79-96: as particle_bounding_boxes, linking a rectangle only to
     neighbourhoods that reach the previous slice."""
     return _bounding_boxes(d_bin_vol, threshold, True)
+
+
+def tamura(arr: np.ndarray) -> float:
+    """particle_detection.jl:110-112: the Tamura coefficient std / mean of a slice (standard
+    deviation with the n - 1 divisor). Evaluated in FP64."""
+    a = np.asarray(arr, dtype=np.float64)
+    return float(a.std(ddof=1) / a.mean())
+
+
+def depth_profile(f: Callable, subvol: np.ndarray) -> np.ndarray:
+    """particle_detection.jl:114-116: ``f`` of every slice of a (slices, rows, columns) block."""
+    return np.array([f(subvol[k]) for k in range(subvol.shape[0])], dtype=np.float64)
+
+
+def getcenterfromslice(arr: np.ndarray) -> Tuple[float, float]:
+    """particle_detection.jl:118-131: the intensity-weighted centre (x, y) of a slice, 1-based.
+    ``arr`` is in the tensor layout (row r = the reference's x = r + 1, column c = its
+    y = c + 1)."""
+    a = np.asarray(arr, dtype=np.float64)
+    total = a.sum()
+    x = float((a.sum(axis=1) * np.arange(1, a.shape[0] + 1)).sum() / total)
+    y = float((a.sum(axis=0) * np.arange(1, a.shape[1] + 1)).sum() / total)
+    return x, y
+
+
+def gaussian_kernel(sigma: float) -> np.ndarray:
+    """The profile smoothing kernel of particle_coordinates, ImageFiltering.jl's
+    ``Kernel.gaussian((sigma,))``: exp(-x^2 / (2 sigma^2)) for x = -w..w, w = 2 ceil(sigma),
+    normalised to sum 1."""
+    w = 2 * int(np.ceil(sigma))
+    x = np.arange(-w, w + 1, dtype=np.float64)
+    g = np.exp(-x * x / (2.0 * sigma * sigma))
+    return g / g.sum()
+
+
+def _smooth_replicate(v: np.ndarray, kernel: np.ndarray) -> np.ndarray:
+    """Correlation with a centred odd kernel, borders replicated (ImageFiltering's default)."""
+    w = kernel.size // 2
+    padded = np.concatenate([np.full(w, v[0]), v, np.full(w, v[-1])])
+    return np.correlate(padded, kernel, mode="valid")
+
+
+def particle_coordinates(particle_bbs: Dict, d_vol: torch.Tensor, depth_metrics: Callable = tamura,
+                         profile_smoothing_kernel: Optional[np.ndarray] = None) -> Dict:
+    """particle_detection.jl:148-160: per neighbourhood [x_min, y_min, z_min, x_max, y_max, z_max]
+    of ``particle_bbs``, the depth is the first maximum of the smoothed ``depth_metrics`` profile
+    over the block's slices and (x, y) the intensity-weighted centre of that slice, all 1-based
+    volume coordinates as float32. ``d_vol`` is an N0f8 volume of shape (slices, ny, nx); each
+    block is copied to the host (it is a few hundred kB at most)."""
+    if d_vol.dtype != torch.uint8 or d_vol.dim() != 3:
+        raise AssertionError("particle_coordinates takes an N0f8 (uint8) volume of shape (slices, ny, nx)")
+    kernel = gaussian_kernel(5.0) if profile_smoothing_kernel is None else np.asarray(profile_smoothing_kernel, np.float64)
+    out = {}
+    for key, (x0, y0, z0, x1, y1, z1) in particle_bbs.items():
+        # the reference's d_vol[y0:y1, x0:x1, z0:z1]: tensor [z, x - 1, y - 1]
+        block = d_vol[z0 - 1:z1, x0 - 1:x1, y0 - 1:y1].cpu().numpy().astype(np.float32) / np.float32(255)
+        profile = _smooth_replicate(depth_profile(depth_metrics, block), kernel).astype(np.float32)
+        z = int(np.argmax(profile)) + 1
+        x, y = getcenterfromslice(block[z - 1])
+        out[key] = [np.float32(x + x0 - 1), np.float32(y + y0 - 1), np.float32(z + z0 - 1)]
+    return out

@@ -19,7 +19,10 @@ resident uint8 stack with one read of every byte.
 """
 from __future__ import annotations
 
-from typing import List, Optional, Sequence, Union
+import json
+import math
+import uuid
+from typing import Dict, List, Optional, Sequence, Union
 
 import numpy as np
 import torch
@@ -28,7 +31,7 @@ from . import _lib
 from ._lib import check
 
 __all__ = ["load_grayimg", "load_gray2float", "make_background", "cu_make_background_mode",
-           "cu_make_background", "cu_ingest_u8"]
+           "cu_make_background", "cu_ingest_u8", "dictsave", "dictload"]
 
 BACKGROUND_OFFSET = 64 / 255  # preprocessings.md:23
 
@@ -157,3 +160,23 @@ def cu_ingest_u8(images: Union[torch.Tensor, np.ndarray], bkg=None, offset: Opti
     check(_lib.load().holo_ingest_u8(_lib.handle(dev).ptr, stream, nx, ny, count, stack.data_ptr(),
                                      b.data_ptr() if b is not None else None, off, out.data_ptr()))
     return out[0] if squeeze else out
+
+
+def _json_f32(v) -> str:
+    f = np.float32(v)
+    return str(f) if math.isfinite(f) else "null"
+
+
+def dictsave(filename: str, d: Dict) -> None:
+    """utils.jl:340-345: a particle dictionary {UUID: [Float32, ...]} as one JSON object, keys the
+    UUID strings, values printed as the shortest Float32 decimals (as Julia prints Float32)."""
+    items = (f"{json.dumps(str(k))}:[{','.join(_json_f32(v) for v in vals)}]" for k, vals in d.items())
+    with open(filename, "w") as io:
+        io.write("{" + ",".join(items) + "}")
+
+
+def dictload(filename: str) -> Dict[uuid.UUID, List[np.float32]]:
+    """utils.jl:358-365: the dictionary written by dictsave, UUID keys, Float32 values."""
+    with open(filename) as io:
+        data = json.load(io)
+    return {uuid.UUID(k): [np.float32(v) for v in vals] for k, vals in data.items()}
