@@ -120,10 +120,12 @@ struct holo_context {
     int nx, ny;
     const void *buf, *i1, *i2, *hf, *hb;
     double zf, zb, wl, dx;
-    int gf, gb, W, rows;
+    int gf, gb, W, rows, ex, ey;
+    int n;  // iterations captured
     bool operator<(const GsKey& o) const {
-      return std::tie(nx, ny, buf, i1, i2, hf, hb, zf, zb, wl, dx, gf, gb, W, rows) <
-             std::tie(o.nx, o.ny, o.buf, o.i1, o.i2, o.hf, o.hb, o.zf, o.zb, o.wl, o.dx, o.gf, o.gb, o.W, o.rows);
+      return std::tie(nx, ny, buf, i1, i2, hf, hb, zf, zb, wl, dx, gf, gb, W, rows, ex, ey, n) <
+             std::tie(o.nx, o.ny, o.buf, o.i1, o.i2, o.hf, o.hb, o.zf, o.zb, o.wl, o.dx, o.gf, o.gb, o.W, o.rows,
+                      o.ex, o.ey, o.n);
     }
   };
   std::map<GsKey, hipGraphExec_t> gs_graphs;
@@ -1280,6 +1282,35 @@ int holo_reconst_vol_and_xyprojection_padded(holo_handle h, void* stream, int nx
   return r;
 }
 
+// The cached graph of `n` identical GS iterations for `key` (captured on first use).
+constexpr int kGsGraphIters = 16;
+int gs_graph(holo_context* h, holo_context::GsKey key, int n, const Geom& g, int Ex, int Ey, GsPair* p,
+             float sc, hipGraphExec_t* out) {
+  key.n = n;
+  auto it = h->gs_graphs.find(key);
+  if (it != h->gs_graphs.end()) {
+    *out = it->second;
+    return HOLO_OK;
+  }
+  hipGraph_t graph = nullptr;
+  HOLO_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
+  int r = HOLO_OK;
+  for (int i = 0; i < n && r == HOLO_OK; ++i) r = gs_round(h, g, Ex, Ey, p, 1, 0, sc, h->cap);
+  const hipError_t e = hipStreamEndCapture(h->cap, &graph);
+  if (r != HOLO_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  HOLO_HIP(e);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HOLO_HIP(ei);
+  h->gs_graphs.emplace(key, exec);
+  *out = exec;
+  return HOLO_OK;
+}
+
 int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const float* holo1,
                          const float* holo2, const holo_tf* fwd, const holo_tf* bwd,
                          int iters, void* out, int use_graph) {
@@ -1308,28 +1339,16 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
   HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, holo1, 0, p.B, 0, 1.0f, 1, st));  // psi1 = sqrt(I1), row FFT
   const int body = iters - 1;  // iterations 0 .. iters-2 are identical; the last one stores psi
   if (use_graph && body > 0) {
+    // A graph holds up to kGsGraphIters iterations (4 launches each): graphs of one iteration
+    // leave the device idle between graph launches.
     holo_context::GsKey key{nx, ny, p.B, holo1, holo2, fwd->data, bwd->data, fwd->z, bwd->z,
-                            fwd->wavlen, fwd->dx, fwd->generated, bwd->generated, g.W, g.rows};
-    hipGraphExec_t exec = nullptr;
-    auto it = h->gs_graphs.find(key);
-    if (it != h->gs_graphs.end()) {
-      exec = it->second;
-    } else {
-      hipGraph_t graph = nullptr;
-      HOLO_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
-      const int r = gs_round(h, g, Ex, Ey, &p, 1, 0, sc, h->cap);
-      const hipError_t e = hipStreamEndCapture(h->cap, &graph);
-      if (r != HOLO_OK) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return r;
-      }
-      HOLO_HIP(e);
-      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      HOLO_HIP(ei);
-      h->gs_graphs.emplace(key, exec);
-    }
-    for (int it2 = 0; it2 < body; ++it2) HOLO_HIP(hipGraphLaunch(exec, st));
+                            fwd->wavlen, fwd->dx, fwd->generated, bwd->generated, g.W, g.rows, Ex, Ey, 0};
+    const int chunk = std::min(body, kGsGraphIters);
+    hipGraphExec_t full = nullptr, rest = nullptr;
+    HOLO_TRY(gs_graph(h, key, chunk, g, Ex, Ey, &p, sc, &full));
+    if (body % chunk) HOLO_TRY(gs_graph(h, key, body % chunk, g, Ex, Ey, &p, sc, &rest));
+    for (int c = 0; c < body / chunk; ++c) HOLO_HIP(hipGraphLaunch(full, st));
+    if (rest) HOLO_HIP(hipGraphLaunch(rest, st));
   } else {
     for (int it2 = 0; it2 < body; ++it2) HOLO_TRY(gs_round(h, g, Ex, Ey, &p, 1, it2, sc, st));
   }

@@ -160,13 +160,16 @@ __global__ void k_lpf_super_gaussian(int n, double sigma, double dx, float* __re
 // helpers shared by the pass kernels
 // ------------------------------------------------------------------------------------------
 // LDS of a row pass: `rows` padded lines, then the x plan's twiddle table.
+// STAGE false leaves the twiddle copy to the kernel, which then issues its global loads
+// first: the copy's LDS writes wait for the table, and a single-plane launch (two waves per
+// SIMD) cannot hide that latency behind other waves.
 struct RowCtx {
   int t, w, j;
   bool ok;
   float2* line;
-  const float2* tw;
+  float2* tw;
 };
-template <int E, int L>
+template <int E, int L, bool STAGE = true>
 __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
   RowCtx c;
   const int T = line_threads<E, L>(g.px);
@@ -176,7 +179,7 @@ __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
   c.ok = c.j < g.ny;
   c.line = lds + c.w * lds_padded_len(g.nx);
   float2* tw = lds + g.rows * lds_padded_len(g.nx);
-  stage_twiddles(tw, g.twx, g.px.ntw);
+  if (STAGE) stage_twiddles(tw, g.twx, g.px.ntw);
   c.tw = tw;
   return c;
 }
@@ -185,7 +188,7 @@ __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
 struct ColCtx {
   int t, w, i;
   float2* col;
-  const float2* tw;
+  float2* tw;
 };
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2) by linear
 // workgroup id. Column tiles are renumbered so that each XCD walks a contiguous range of
@@ -197,7 +200,7 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
 }
 
-template <int E, int L>
+template <int E, int L, bool STAGE = true>
 __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   ColCtx c;
   c.w = threadIdx.x & (g.W - 1);
@@ -205,7 +208,7 @@ __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   c.i = xcd_tile(blockIdx.x, gridDim.x) * g.W + c.w;
   c.col = lds + c.w;
   float2* tw = lds + g.W * lds_padded_len(g.ny);
-  stage_twiddles(tw, g.twy, g.py.ntw);
+  if (STAGE) stage_twiddles(tw, g.twy, g.py.ntw);
   c.tw = tw;
   return c;
 }
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
   const GsDesc& pd = gt.d[HMODE == 2 ? blockIdx.z : 0];
   const float2* __restrict__ src = HMODE == 2 ? pd.B : a.in;
   float2* __restrict__ dst = HMODE == 2 ? pd.B : a.out;
-  const ColCtx c = col_ctx<E, L>(g, lds);
+  const ColCtx c = col_ctx<E, L, false>(g, lds);
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
@@ -584,6 +587,17 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
 #pragma unroll
   for (int e = 0; e < E; ++e)
     v[e] = ok ? src[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
+  // the transfer (or filter) values are loaded with the column, in the same memory latency
+  constexpr bool kHArr = HMODE == 1 || HMODE == 2;
+  float2 hv[kHArr ? E : 1];
+  float lv[HMODE == 3 ? E : 1];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if constexpr (HMODE == 1) hv[e] = a.H[centred_index(ci, c.t + T * e, g.nx, g.ny)];
+    if constexpr (HMODE == 2) hv[e] = pd.H[static_cast<size_t>(c.t + T * e) * g.nx + ci];
+    if constexpr (HMODE == 3) lv[e] = a.lpf[centred_index(ci, c.t + T * e, g.nx, g.ny)];
+  }
+  stage_twiddles(c.tw, g.twy, g.py.ntw);
   fft_run<E, -1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if constexpr (HMODE == 0) {
     const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
@@ -596,20 +610,13 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
     }
   } else if constexpr (HMODE == 1) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = cmul(v[e], a.H[centred_index(ci, c.t + T * e, g.nx, g.ny)]);
+    for (int e = 0; e < E; ++e) v[e] = cmul(v[e], hv[e]);
   } else if constexpr (HMODE == 2) {
-    const float2* __restrict__ H = pd.H;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float2 h = H[static_cast<size_t>(c.t + T * e) * g.nx + ci];
-      v[e] = pd.conj ? cmulconj(v[e], h) : cmul(v[e], h);
-    }
+    for (int e = 0; e < E; ++e) v[e] = pd.conj ? cmulconj(v[e], hv[e]) : cmul(v[e], hv[e]);
   } else {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float l = a.lpf[centred_index(ci, c.t + T * e, g.nx, g.ny)];
-      v[e] = make_float2(l * v[e].x, l * v[e].y);
-    }
+    for (int e = 0; e < E; ++e) v[e] = make_float2(lv[e] * v[e].x, lv[e] * v[e].y);
   }
   fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if (ok) {
@@ -630,16 +637,21 @@ __global__ __launch_bounds__(512) void k_gs_rows(GsTable gt, float sc, Geom g) {
   const float* __restrict__ I = rd.I;
   const bool final = rd.final != 0;
   float2* __restrict__ out = final ? rd.out : rd.B;
-  const RowCtx c = row_ctx<E, L>(g, lds);
+  const RowCtx c = row_ctx<E, L, false>(g, lds);
   const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
   float2 v[E];
+  float iv[E];  // the intensity is loaded with the row, in the same memory latency
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = c.ok ? U[rowoff + c.t + T * e] : make_float2(0.f, 0.f);
+  for (int e = 0; e < E; ++e) {
+    v[e] = c.ok ? U[rowoff + c.t + T * e] : make_float2(0.f, 0.f);
+    iv[e] = c.ok ? I[rowoff + c.t + T * e] : 0.f;
+  }
+  stage_twiddles(c.tw, g.twx, g.px.ntw);
   fft_run<E, +1, L>(v, c.line, 1, c.t, g.px, c.tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const float A = c.ok ? sqrtf(I[rowoff + c.t + T * e]) : 0.f;
+    const float A = sqrtf(iv[e]);
     const float a = v[e].x * sc, b = v[e].y * sc;
     const float r2 = a * a + b * b;
     if (r2 >= 1e-30f) {  // v_rsq_f32 (1 ulp) away from the denormal range
