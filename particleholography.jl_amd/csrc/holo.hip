@@ -116,16 +116,16 @@ struct holo_context {
   size_t batch_ws_limit = 0;                      // batched calls' workspace cap (0 = default)
   hipStream_t cap = nullptr;
   double2* partials = nullptr;  // 256 partial sums for cu_2d_pad's mean
+  // A GS graph reads and writes only the workspace (field, transfer tables, copies of the two
+  // intensities), so its key holds no caller pointer and one graph serves every call of a size.
   struct GsKey {
     int nx, ny;
-    const void *buf, *i1, *i2, *hf, *hb;
-    double zf, zb, wl, dx;
-    int gf, gb, W, rows, ex, ey;
+    const void *buf, *hb;
+    int conj, W, rows, ex, ey;
     int n;  // iterations captured
     bool operator<(const GsKey& o) const {
-      return std::tie(nx, ny, buf, i1, i2, hf, hb, zf, zb, wl, dx, gf, gb, W, rows, ex, ey, n) <
-             std::tie(o.nx, o.ny, o.buf, o.i1, o.i2, o.hf, o.hb, o.zf, o.zb, o.wl, o.dx, o.gf, o.gb, o.W, o.rows,
-                      o.ex, o.ey, o.n);
+      return std::tie(nx, ny, buf, hb, conj, W, rows, ex, ey, n) <
+             std::tie(o.nx, o.ny, o.buf, o.hb, o.conj, o.W, o.rows, o.ex, o.ey, o.n);
     }
   };
   std::map<GsKey, hipGraphExec_t> gs_graphs;
@@ -733,7 +733,8 @@ int gs_round(holo_context* h, const Geom& g, int Ex, int Ey, const GsPair* pairs
     f.d[k] = GsDesc{p.B, p.hf, nullptr, nullptr, 0, 0};
     r2.d[k] = GsDesc{p.B, nullptr, p.I2, nullptr, 0, 0};
     b.d[k] = GsDesc{p.B, p.hb, nullptr, nullptr, p.conj_b, 0};
-    r1.d[k] = GsDesc{p.B, nullptr, p.I1, p.out, 0, it + 1 == p.iters ? 1 : 0};
+    const bool last = it + 1 == p.iters;
+    r1.d[k] = GsDesc{p.B, nullptr, p.I1, last ? p.out : nullptr, 0, last ? 1 : 0};
     f.n = r2.n = b.n = r1.n = k + 1;
   }
   if (f.n == 0) return HOLO_OK;
@@ -863,9 +864,7 @@ int holo_set_tuning(holo_handle h, int batch, int spg_cols, int spg_rows, int co
   h->tune.spg_cols = spg_cols;
   h->tune.spg_rows = spg_rows;
   h->tune.cols = cols_per_wg;
-  h->tune.rows = rows_per_wg;
-  for (auto& kv : h->gs_graphs) (void)hipGraphExecDestroy(kv.second);
-  h->gs_graphs.clear();
+  h->tune.rows = rows_per_wg;  // cached GS graphs are keyed by their tiles and stay valid
   return HOLO_OK;
 }
 
@@ -1282,8 +1281,11 @@ int holo_reconst_vol_and_xyprojection_padded(holo_handle h, void* stream, int nx
   return r;
 }
 
-// The cached graph of `n` identical GS iterations for `key` (captured on first use).
+// The cached graph of `n` identical GS iterations for `key` (captured on first use). Keys
+// change only with the size, the tiles and the workspace address; past kGsGraphCache graphs
+// (workspace reallocated many times) the cache is emptied once the device is idle.
 constexpr int kGsGraphIters = 16;
+constexpr size_t kGsGraphCache = 64;
 int gs_graph(holo_context* h, holo_context::GsKey key, int n, const Geom& g, int Ex, int Ey, GsPair* p,
              float sc, hipGraphExec_t* out) {
   key.n = n;
@@ -1291,6 +1293,11 @@ int gs_graph(holo_context* h, holo_context::GsKey key, int n, const Geom& g, int
   if (it != h->gs_graphs.end()) {
     *out = it->second;
     return HOLO_OK;
+  }
+  if (h->gs_graphs.size() >= kGsGraphCache) {
+    HOLO_HIP(hipDeviceSynchronize());
+    for (auto& kv : h->gs_graphs) (void)hipGraphExecDestroy(kv.second);
+    h->gs_graphs.clear();
   }
   hipGraph_t graph = nullptr;
   HOLO_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
@@ -1325,14 +1332,19 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
   const Tuning gt = gs_tiles(h, nx, ny);
   HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &gt));
   const size_t plane = static_cast<size_t>(nx) * ny;
-  HOLO_TRY(ensure_ws(h, plane * sizeof(float2) * 3));  // [field | H fwd | H bwd]
+  // workspace: [field | H fwd | H bwd | I1 | I2]
+  HOLO_TRY(ensure_ws(h, plane * (3 * sizeof(float2) + 2 * sizeof(float))));
   GsPair p{};
   p.B = static_cast<float2*>(h->ws);
-  p.I1 = holo1;
-  p.I2 = holo2;
+  float2* hf = p.B + plane;
+  float* i1c = reinterpret_cast<float*>(hf + 2 * plane);
+  float* i2c = i1c + plane;
+  HOLO_HIP(hipMemcpyAsync(i1c, holo1, plane * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HOLO_HIP(hipMemcpyAsync(i2c, holo2, plane * sizeof(float), hipMemcpyDeviceToDevice, st));
+  p.I1 = i1c;
+  p.I2 = i2c;
   p.out = static_cast<float2*>(out);
   p.iters = iters;
-  float2* hf = p.B + plane;
   p.hf = hf;
   HOLO_TRY(gs_transfers(g, fwd, bwd, hf, hf + plane, &p.hb, &p.conj_b, st));
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
@@ -1341,8 +1353,7 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
   if (use_graph && body > 0) {
     // A graph holds up to kGsGraphIters iterations (4 launches each): graphs of one iteration
     // leave the device idle between graph launches.
-    holo_context::GsKey key{nx, ny, p.B, holo1, holo2, fwd->data, bwd->data, fwd->z, bwd->z,
-                            fwd->wavlen, fwd->dx, fwd->generated, bwd->generated, g.W, g.rows, Ex, Ey, 0};
+    holo_context::GsKey key{nx, ny, p.B, p.hb, p.conj_b, g.W, g.rows, Ex, Ey, 0};
     const int chunk = std::min(body, kGsGraphIters);
     hipGraphExec_t full = nullptr, rest = nullptr;
     HOLO_TRY(gs_graph(h, key, chunk, g, Ex, Ey, &p, sc, &full));

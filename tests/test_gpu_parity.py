@@ -298,6 +298,24 @@ def test_gs_graph_equals_eager(gpu):
         torch.cuda.synchronize()
 
 
+def test_gs_graphs_reused_across_inputs_and_iteration_counts(gpu):
+    """The cached graphs hold only workspace addresses: new input tensors, other iteration
+    counts (graphs of 16 iterations plus a remainder) and another transfer pair replay them and
+    still equal the eager path bit for bit."""
+    ph = _ph()
+    n, z = 128, 2000.0
+    dsq = ph.cu_transfer_sqrt_arr(n, LAM, DX)
+    mirrored = (ph.cu_transfer(z, n, LAM, dsq), ph.cu_transfer(-z, n, LAM, dsq))
+    h_other = ph.cu_transfer(-1500.0, n, LAM, dsq)
+    arrays = (ph.CuTransfer(mirrored[0].data.clone()), ph.CuTransfer(h_other.data.clone()))  # not mirrored
+    for seed, iters, (hf, hb) in [(5, 7, mirrored), (6, 40, mirrored), (7, 18, arrays), (8, 40, arrays), (9, 7, mirrored)]:
+        i1, i2 = _gs_case(n, seed, z)
+        t1, t2 = _t(i1, gpu), _t(i2, gpu)
+        a = ph.cu_phase_retrieval_holo(t1, t2, hf, hb, iters, n, use_graph=False).data.cpu().numpy()
+        b = ph.cu_phase_retrieval_holo(_t(i1, gpu), _t(i2, gpu), hf, hb, iters, n, use_graph=True).data.cpu().numpy()
+        np.testing.assert_array_equal(a, b)
+
+
 def test_gs_arrays_vs_oracle(gpu):
     ph = _ph()
     n, z = 64, 2000.0
