@@ -81,8 +81,10 @@ bool dispatch_E(int E, F&& f) {
   }
 }
 
-// Lengths with a compile-time plan (the configured hologram sides); others use the runtime
-// plan. f receives (integral_constant<E>, integral_constant<L or 0>).
+// Lengths with a compile-time plan (the configured hologram sides at their default E, and the
+// Gerchberg-Saxton line shapes); others use the runtime plan (512^2 GS: 32.5k iterations/s on
+// runtime plans, 42.0k on compile-time ones). f receives (integral_constant<E>,
+// integral_constant<L or 0>).
 template <class F>
 bool dispatch_EL(int E, int L, F&& f) {
 #define HOLO_L(l, e)                                                          \
@@ -91,7 +93,7 @@ bool dispatch_EL(int E, int L, F&& f) {
     return true;                                                              \
   }
   HOLO_L(256, 16) HOLO_L(512, 16) HOLO_L(1024, 16) HOLO_L(2048, 16) HOLO_L(768, 12) HOLO_L(1280, 20)
-  HOLO_L(1024, 8)
+  HOLO_L(1024, 8) HOLO_L(1024, 4) HOLO_L(512, 8) HOLO_L(512, 4) HOLO_L(2048, 8)
 #undef HOLO_L
   return dispatch_E(E, [&](auto ec) { f(ec, std::integral_constant<int, 0>{}); });
 }
@@ -284,13 +286,13 @@ Tuning eff_tuning(const holo_context* h, int nx, int ny) {
 // Tiles of the Gerchberg-Saxton passes. A single pair is one plane per launch: at the
 // reconstruction's 16 elements per thread a 1024^2 row pass is one wave per SIMD, so GS runs
 // its lines with fewer elements per thread (more waves, one more LDS exchange) and narrower
-// tiles. Measured on MI355X (iterations/s over the generic choice): 1024^2 E 8/8, 4 columns,
-// 1 row +15%; 512^2 E 4/8, 4 x 1 +8%; 2048^2 E 8 rows, 4 x 1 +5%. Global tuning
-// (holo_set_tuning) still sets the tile sizes.
+// tiles. Measured on MI355X (E rows / columns, iterations/s): 1024^2 E 4/8 on 4-column and
+// 1-row tiles 20.9k (16.2k at the reconstruction's E 16/16, 20.3k at E 8/8); 512^2 E 4/8, 4 x 1
+// 42.0k; 2048^2 E 8/16, 4 x 1 8.7k. Global tuning (holo_set_tuning) still sets the tile sizes.
 Tuning gs_tiles(const holo_context* h, int nx, int ny) {
   Tuning t;
   if (nx == 1024 && ny == 1024) {
-    t.erow = 8;
+    t.erow = 4;
     t.ecol = 8;
     t.cols = 4;
     t.rows = 1;
