@@ -322,8 +322,9 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
   auto fewer = [](int want, int L) {
     return (want > 0 && L % want == 0 && L / want <= 512 && want < choose_E(L)) ? want : 0;
   };
-  HOLO_TRY(get_plan(h, nx, &px, fewer(tiles ? tiles->erow : 0, nx)));
-  HOLO_TRY(get_plan(h, ny, &py, fewer(tiles && tiles->ecol ? tiles->ecol : h->col_e, ny)));
+  const Tuning tn = tiles ? *tiles : eff_tuning(h, nx, ny);
+  HOLO_TRY(get_plan(h, nx, &px, fewer(tn.erow, nx)));
+  HOLO_TRY(get_plan(h, ny, &py, fewer(tn.ecol ? tn.ecol : h->col_e, ny)));
   std::memset(g, 0, sizeof(Geom));
   g->nx = nx;
   g->ny = ny;
@@ -336,7 +337,6 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
   const size_t colbytes = static_cast<size_t>(lds_padded_len(ny)) * sizeof(float2);
   const size_t coltw = static_cast<size_t>(py->lp.ntw) * sizeof(float2);
   const size_t maxl = static_cast<size_t>(h->max_lds);
-  const Tuning tn = tiles ? *tiles : eff_tuning(h, nx, ny);
   int W = tn.cols;
   if (W <= 0) {
     W = 16;
@@ -813,7 +813,7 @@ int holo_create(holo_handle* out, int device) {
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
-      int nx, ny, batch, spg_cols, spg_rows, cols, rows;
+      int nx, ny, batch, spg_cols, spg_rows, cols, rows, erow = 0, ecol = 0;
     };
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
                                        {768, 768, 56, 16, 4, 4, 2},
@@ -826,6 +826,8 @@ int holo_create(holo_handle* out, int device) {
       t.spg_rows = e.spg_rows;
       t.cols = e.cols;
       t.rows = e.rows;
+      t.erow = e.erow;
+      t.ecol = e.ecol;
       h->size_tune[{e.nx, e.ny}] = t;
     }
   }
