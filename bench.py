@@ -99,7 +99,7 @@ def main():
         ph.reconst_xyprojection_holo(holo, h0, hdz, SLICES, out=proj)
 
     if args.profile_detect:
-        r = bench_detect(ph, torch, holo, reps=args.steps)
+        r = bench_detect(ph, torch, dev, reps=args.steps)
         log("bench: detection profile run done")
         print(json.dumps({"profile_detect": True, "steps": args.steps, **r}), flush=True)
         return
@@ -183,7 +183,7 @@ def main():
         extras["gs_1024"] = bench_gs(ph, synth, torch, dev, d_sqr)
         extras["lpf_1024"] = bench_lpf(ph, torch, holo)
         extras["ingest_1024"] = bench_ingest(ph, torch, dev)
-        extras["detect_1024x1000"] = bench_detect(ph, torch, holo)
+        extras["detect_reference_holograms"] = bench_detect(ph, torch, dev)
         extras["batched_64"] = bench_batched(torch, dev, args.batch_holograms)
         extras["serving"] = bench_serving(torch, args.serve_requests)
 
@@ -277,36 +277,60 @@ def bench_ingest(ph, torch, dev):
             "config": "1024^2 uint8 holograms: 64 ingested with a Float64 background, 256 reduced"}
 
 
-def bench_detect(ph, torch, holo, reps=5):
-    """Particle detection downstream of reconstruction: the 1024^2 x 1000 N0f8 volume of the
-    headline hologram, thresholded at 30/255 inside the labelling kernel, 8-connected
-    components and their rectangles for every slice (one libholo call; 1 B read per voxel plus
-    the union-find state), then the host z-linking into neighbourhoods."""
+def bench_detect(ph, torch, dev, reps=5):
+    """Particle detection on the reference's own test holograms (tests/golden/holo1.png and
+    holo2.png, runtests.jl:153-193): GS 9 iterations at +-8e4 um, 1000 N0f8 slices of 1024^2
+    from 8e4 um at 100 um, threshold 10/255 fused into the labelling. Timed: components and
+    rectangles of every slice (one libholo call, 1 B read per voxel plus the union-find state),
+    particle_bounding_boxes (that call, the copy of the used rectangles and the native z-linking),
+    particle_coordinates, and the whole chain from the two 8-bit holograms."""
     from particleholography import _lib
 
-    d_sqr = ph.cu_transfer_sqrt_arr(N, LAM, DX)
-    h0, hdz = ph.cu_transfer(-Z0, N, LAM, d_sqr), ph.cu_transfer(-DZ, N, LAM, d_sqr)
-    vol = ph.cu_get_reconst_vol(ph.cu_gabor_wavefront(holo), h0, hdz, SLICES)
+    golden = ROOT / "tests" / "golden"
+    lam, dx, z0, dz, n, slices, pr_dist, pr_iter, thr = 0.6328, 10.0, 8e4, 100.0, 1024, 1000, 8e4, 9, 10 / 255
+    i1 = torch.from_numpy(ph.load_gray2float(str(golden / "holo1.png"))).to(dev)
+    i2 = torch.from_numpy(ph.load_gray2float(str(golden / "holo2.png"))).to(dev)
+    d_sqr = ph.cu_transfer_sqrt_arr(n, lam, dx)
+    d_tf, d_slice = ph.cu_transfer(-z0, n, lam, d_sqr), ph.cu_transfer(-dz, n, lam, d_sqr)
+    d_pr, d_pr_inv = ph.cu_transfer(pr_dist, n, lam, d_sqr), ph.cu_transfer(-pr_dist, n, lam, d_sqr)
+
+    def chain():
+        d_holo = ph.cu_phase_retrieval_holo(i1, i2, d_pr, d_pr_inv, pr_iter, n)
+        vol = ph.cu_get_reconst_vol(d_holo, d_tf, d_slice, slices)
+        pbbs = ph.particle_bounding_boxes(vol, threshold=thr)
+        return vol, pbbs, ph.particle_coordinates(pbbs, vol)
+
+    chain()  # warm-up (plans, graphs, workspace)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vol, pbbs, coords = chain()
+    chain_s = time.perf_counter() - t0
     max_boxes = 4096
-    boxes = torch.empty((SLICES, max_boxes, 5), dtype=torch.int32, device=vol.device)
-    nbox = torch.empty(SLICES, dtype=torch.int32, device=vol.device)
+    boxes = torch.empty((slices, max_boxes, 5), dtype=torch.int32, device=vol.device)
+    nbox = torch.empty(slices, dtype=torch.int32, device=vol.device)
     lib, h = _lib.load(), _lib.handle(0)
 
     def run():
         st = torch.cuda.current_stream().cuda_stream
-        _lib.check(lib.holo_ccl_boxes(h.ptr, st, N, N, SLICES, vol.data_ptr(), _lib.HOLO_CCL_N0F8_LE,
-                                      30 / 255, max_boxes, boxes.data_ptr(), nbox.data_ptr()))
+        _lib.check(lib.holo_ccl_boxes(h.ptr, st, n, n, slices, vol.data_ptr(), _lib.HOLO_CCL_N0F8_LE,
+                                      thr, max_boxes, boxes.data_ptr(), nbox.data_ptr()))
 
     _, sec = timed_steps(run, reps, 1, torch)
     t0 = time.perf_counter()
-    pbbs = ph.particle_bounding_boxes(vol, threshold=30 / 255)
-    total = time.perf_counter() - t0
+    pbbs = ph.particle_bounding_boxes(vol, threshold=thr)
+    link_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ph.particle_coordinates(pbbs, vol)
+    coord_s = time.perf_counter() - t0
     return {"label_boxes_ms": round(sec / reps * 1e3, 3),
-            "slices_per_s": round(SLICES * reps / sec, 1),
-            "voxel_GBs": round(float(N * N) * SLICES * reps / sec / 1e9, 1),
-            "components": int(nbox.sum().item()), "neighbourhoods": len(pbbs),
-            "particle_bounding_boxes_ms": round(total * 1e3, 2),
-            "config": "1024^2 x 1000 N0f8 volume, threshold 30/255, 8-connected, max 4096 boxes per slice"}
+            "slices_per_s": round(slices * reps / sec, 1),
+            "voxel_GBs": round(float(n * n) * slices * reps / sec / 1e9, 1),
+            "components": int(nbox.sum().item()), "neighbourhoods": len(pbbs), "particles": len(coords),
+            "particle_bounding_boxes_ms": round(link_s * 1e3, 2),
+            "particle_coordinates_ms": round(coord_s * 1e3, 2),
+            "holograms_to_particles_ms": round(chain_s * 1e3, 2),
+            "config": "reference test holograms holo1/holo2 (1024^2): GS 9 iterations at +-8e4 um, 1000 N0f8 "
+                      "slices from 8e4 um at 100 um, threshold 10/255, 8-connected, max 4096 boxes per slice"}
 
 
 def bench_batched(torch, dev, nholo, autotune=True, reps=1):

@@ -214,6 +214,18 @@ int holo_label_boxes(holo_handle h, void* stream, int nx, int ny, int count, con
                      int max_boxes, holo_box* boxes, int* nboxes);
 /* out = 3 x 3 in-plane dilation of each slice (uint8 / Bool), false on the slice border. */
 int holo_dilate(holo_handle h, void* stream, int nx, int ny, int count, const uint8_t* in, uint8_t* out);
+/* Host-only (no device work, no handle): the z-linking of particle_bounding_boxes(_3d)
+ * (particle_detection.jl:58-96): gen_particle_neighborhoods on slice 1 (ccl.jl:266-277), one
+ * update_particle_neighborhoods! / update_particle_neighborhoods3d! per further slice
+ * (ccl.jl:328-409, three_d selects), then finalize_particle_neighborhoods! (ccl.jl:424-466).
+ * rects: host int32 (x_min, y_min, x_max, y_max) of every slice's rectangles in order, slice s
+ * (1-based) at rects[4*offsets[s-1] .. 4*offsets[s]); offsets has nslices + 1 entries. Every
+ * neighbourhood ever started is reported in start order (capacity offsets[nslices] rows):
+ * rows (int64 x_min, y_min, z_min, x_max, y_max, z_max, final values), born (its start slice),
+ * alive (1 if it survives the unit-cube deletions and finalize); *nrows their number. The
+ * reference's dict keys are the caller's: one key per started neighbourhood, in this order. */
+int holo_link_rectangles(const int32_t* rects, const int64_t* offsets, int nslices, int three_d,
+                         int64_t* rows, int32_t* born, uint8_t* alive, int64_t* nrows);
 
 /* ---- batched reconstruction (no counterpart in the reference: its functions take one
  *      hologram; this runs many cu_get_reconst_xyprojection calls, holofunc.jl:248-266, together) */

@@ -21,6 +21,7 @@
 #include "holo_kernels.hpp"
 #include "ingest_kernels.hpp"
 #include "ccl_kernels.hpp"
+#include "link_host.hpp"
 
 using namespace holo;
 
@@ -1223,6 +1224,45 @@ int holo_dilate(holo_handle h, void* stream, int nx, int ny, int count, const ui
     hipLaunchKernelGGL(k_dilate3x3, dim3((nx + 255) / 256, ny, c), dim3(256), 0, st, nx, ny, c, in + off, out + off);
     HOLO_TRY(check_launch());
   }
+  return HOLO_OK;
+}
+
+int holo_link_rectangles(const int32_t* rects, const int64_t* offsets, int nslices, int three_d,
+                         int64_t* rows, int32_t* born, uint8_t* alive, int64_t* nrows) {
+  if (!nrows || (nslices > 0 && !offsets)) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (nslices < 0) return fail(HOLO_ERR_ARG, "nslices must be >= 0");
+  *nrows = 0;
+  if (nslices == 0) return HOLO_OK;
+  if (offsets[0] != 0) return fail(HOLO_ERR_ARG, "offsets[0] must be 0");
+  for (int s = 0; s < nslices; ++s)
+    if (offsets[s + 1] < offsets[s]) return fail(HOLO_ERR_ARG, "offsets must not decrease");
+  const int64_t total = offsets[nslices];
+  if (total > 0 && (!rects || !rows || !born || !alive)) return fail(HOLO_ERR_ARG, "NULL argument");
+  LinkTable t;
+  t.reserve(static_cast<size_t>(total));
+  LinkGrid grid_store;
+  const bool use_grid = !three_d && grid_store.init(total, [&](int64_t q, int32_t* r) {
+    for (int c = 0; c < 4; ++c) r[c] = rects[4 * q + c];
+  });
+  LinkGrid* grid = use_grid ? &grid_store : nullptr;
+  for (int64_t q = 0; q < offsets[1]; ++q) link_start(t, grid, rects + 4 * q, 1);
+  t.rebuild_live();
+  std::vector<uint32_t> sub, touched;
+  for (int s = 2; s <= nslices; ++s)
+    link_update(t, grid, rects + 4 * offsets[s - 1], offsets[s] - offsets[s - 1], s, three_d != 0, sub, touched);
+  link_finalize(t);
+  for (size_t i = 0; i < t.size(); ++i) {
+    int64_t* o = rows + 6 * i;
+    o[0] = t.x0[i];
+    o[1] = t.y0[i];
+    o[2] = t.z0[i];
+    o[3] = t.x1[i];
+    o[4] = t.y1[i];
+    o[5] = t.z1[i];
+    born[i] = t.born[i];
+    alive[i] = t.dead[i] ? 0 : 1;
+  }
+  *nrows = static_cast<int64_t>(t.size());
   return HOLO_OK;
 }
 

@@ -7,11 +7,14 @@ module ParticleHolographyHIP
 
 using AMDGPU
 using FixedPointNumbers
+using Random
+using UUIDs
 
 export CuTransferSqrtPart, CuTransfer, CuWavefront, CuLowPassFilter
 export cu_rectangle_filter, cu_super_gaussian_filter, cu_apply_low_pass_filter, cu_apply_low_pass_filter!
 export cu_make_background_mode, cu_ingest_u8
-export cu_connected_component_labeling, cu_find_valid_labels, slice_bounding_rectangles, cu_dilate
+export cu_connected_component_labeling, cu_find_valid_labels, slice_bounding_rectangles, cu_dilate,
+       link_rectangles, particle_bounding_boxes, particle_bounding_boxes_3d
 export cu_transfer_sqrt_arr, cu_transfer, cu_gabor_wavefront, cu_phase_retrieval_holo,
        cu_get_reconst_vol, cu_get_reconst_xyprojection, cu_get_reconst_vol_and_xyprojection,
        cu_get_reconst_complex_vol, cu_asm_prop!, cu_2d_pad, cu_get_reconst_vol_and_xyprojection_padded
@@ -338,6 +341,50 @@ function slice_bounding_rectangles(vol::ROCArray{T,3}; threshold=nothing, max_bo
         max_boxes = maximum(n)
     end
 end
+
+# gen_particle_neighborhoods + update_particle_neighborhoods(3d)! per slice +
+# finalize_particle_neighborhoods! (ccl.jl:266-466) in libholo's native linking; keys drawn as
+# the reference draws them, uuid1 of a fresh MersenneTwister(1234) per start slice.
+function link_rectangles(rects::Vector{<:AbstractVector}; three_d::Bool=false)
+    nslices = length(rects)
+    nslices == 0 && return Dict{UUID,Vector{Int}}()
+    offsets = zeros(Int64, nslices + 1)
+    for s in 1:nslices
+        offsets[s+1] = offsets[s] + length(rects[s])
+    end
+    total = offsets[end]
+    flat = Vector{Int32}(undef, 4 * total)
+    k = 0
+    for r in rects, br in r, c in 1:4
+        k += 1
+        flat[k] = br[c]
+    end
+    cap = max(total, 1)
+    rows = Matrix{Int64}(undef, 6, cap)
+    born = Vector{Int32}(undef, cap)
+    alive = Vector{UInt8}(undef, cap)
+    nrows = Ref{Int64}(0)
+    _check(ccall((:holo_link_rectangles, libholo), Cint,
+                 (Ptr{Int32}, Ptr{Int64}, Cint, Cint, Ptr{Int64}, Ptr{Int32}, Ptr{UInt8}, Ptr{Int64}),
+                 flat, offsets, nslices, three_d ? 1 : 0, rows, born, alive, nrows))
+    out = Dict{UUID,Vector{Int}}()
+    rng, cur = MersenneTwister(1234), Int32(0)
+    for i in 1:nrows[]
+        if born[i] != cur
+            rng, cur = MersenneTwister(1234), born[i]
+        end
+        key = uuid1(rng)
+        alive[i] == 1 && (out[key] = Vector{Int}(rows[:, i]))
+    end
+    return out
+end
+
+# particle_detection.jl:58-96 with the per-slice rectangles from one libholo call and the
+# native linking; `threshold` fuses d_vol .<= threshold for an N0f8 volume.
+particle_bounding_boxes(d_bin_vol::ROCArray{T,3}; threshold=nothing) where {T} =
+    link_rectangles(slice_bounding_rectangles(d_bin_vol; threshold=threshold); three_d=false)
+particle_bounding_boxes_3d(d_bin_vol::ROCArray{T,3}; threshold=nothing) where {T} =
+    link_rectangles(slice_bounding_rectangles(d_bin_vol; threshold=threshold); three_d=true)
 
 function cu_dilate(vol::ROCArray{Bool,3}; blocksize=32)
     nx, ny, slices = size(vol)

@@ -80,6 +80,7 @@ SIGNATURES = {
     "holo_ccl_boxes": [_vp, _vp, _i, _i, _i, _vp, _i, _d, _i, _vp, _vp],
     "holo_label_boxes": [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp],
     "holo_dilate": [_vp, _vp, _i, _i, _i, _vp, _vp],
+    "holo_link_rectangles": [_vp, _vp, _i, _i, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int64)],
     "holo_reconst_batch": [_vp, _vp, ctypes.POINTER(holo_job), _i],
     "holo_phase_retrieval_batch": [_vp, _vp, ctypes.POINTER(holo_gs_job), _i],
     "holo_set_tuning_for_size": [_vp, _i, _i, _i, _i, _i, _i, _i],

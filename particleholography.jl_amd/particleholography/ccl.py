@@ -17,6 +17,7 @@ memory), so a rectangle's x comes from the tensor's row index and y from its col
 """
 from __future__ import annotations
 
+import ctypes
 import math
 import random
 import uuid
@@ -31,7 +32,7 @@ from ._lib import check
 __all__ = ["cu_connected_component_labeling", "count_labels", "cu_find_valid_labels",
            "get_bounding_rectangles", "gen_particle_neighborhoods", "update_particle_neighborhoods_",
            "update_particle_neighborhoods3d_", "finalize_particle_neighborhoods_", "judge_overlap2d",
-           "judge_overlap3d", "new_rect", "delete_duplicates_", "link_slices"]
+           "judge_overlap3d", "new_rect", "delete_duplicates_", "link_slices", "link_arrays"]
 
 _TYPEMAX, _TYPEMIN = 2 ** 63 - 1, -(2 ** 63)
 
@@ -130,9 +131,26 @@ def get_bounding_rectangles(labels, valid_labels: Sequence[int]) -> List[Tuple[i
 # ------------------------------------------------------------------------------------------
 # particle neighbourhoods (ccl.jl:266-466), host logic
 # ------------------------------------------------------------------------------------------
+_KEY_DRAWS: List[Tuple[int, int]] = []  # (node, clock_seq) draws of random.Random(1234), in order
+_KEY_RNG = random.Random(1234)
+
+
+def _key_draw(k: int) -> Tuple[int, int]:
+    while len(_KEY_DRAWS) <= k:
+        _KEY_DRAWS.append((_KEY_RNG.getrandbits(48) | (1 << 40), _KEY_RNG.getrandbits(14)))
+    return _KEY_DRAWS[k]
+
+
 def _uuid_source():
-    rng = random.Random(1234)  # the reference seeds a fresh MersenneTwister(1234) per call
-    return lambda: uuid.uuid1(node=rng.getrandbits(48) | (1 << 40), clock_seq=rng.getrandbits(14))
+    """A key source replaying one seeded sequence: the reference seeds a fresh
+    MersenneTwister(1234) per call, so every source starts with the same draws (cached here,
+    seeding costs more than the key)."""
+    count = iter(range(1 << 62))
+
+    def new_key():
+        node, seq = _key_draw(next(count))
+        return uuid.uuid1(node=node, clock_seq=seq)
+    return new_key
 
 
 def gen_particle_neighborhoods(bounding_rectangles, slicenum: int) -> Dict[uuid.UUID, List[int]]:
@@ -267,18 +285,46 @@ def update_particle_neighborhoods3d_(particle_neighborhoods, bounding_rectangles
     t.store()
 
 
-def link_slices(per_slice_rectangles, three_d: bool = False):
+def link_arrays(rects: np.ndarray, offsets: np.ndarray, three_d: bool = False) -> Dict[uuid.UUID, List[int]]:
     """gen_particle_neighborhoods on slice 1, then one update per further slice, then
-    finalize_particle_neighborhoods! (particle_detection.jl:58-96), on one table."""
+    finalize_particle_neighborhoods! (particle_detection.jl:58-96), in libholo's native
+    linking (``holo_link_rectangles``). ``rects``: int32 (R, 4) rows (x_min, y_min, x_max,
+    y_max) of every slice in order; ``offsets``: int64 (slices + 1,), slice s (1-based) holding
+    rows offsets[s-1]:offsets[s]. Keys: one UUID source per start slice, as the reference draws a
+    fresh MersenneTwister(1234) per call; the dict holds the survivors in start order."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    nslices = offsets.size - 1
+    if nslices <= 0:
+        return {}
+    rects = np.ascontiguousarray(rects, dtype=np.int32).reshape(-1, 4)
+    total = int(offsets[-1])
+    if rects.shape[0] < total:
+        raise AssertionError("fewer rectangles than offsets[-1]")
+    cap = max(total, 1)
+    rows = np.empty((cap, 6), np.int64)
+    born = np.empty(cap, np.int32)
+    alive = np.empty(cap, np.uint8)
+    n = ctypes.c_int64()
+    check(_lib.load().holo_link_rectangles(rects.ctypes.data, offsets.ctypes.data, nslices, int(bool(three_d)),
+                                           rows.ctypes.data, born.ctypes.data, alive.ctypes.data, ctypes.byref(n)))
+    pn: Dict[uuid.UUID, List[int]] = {}
+    cur, new_key = None, None
+    for i in np.nonzero(alive[:n.value])[0]:
+        if born[i] != cur:
+            cur, new_key = born[i], _uuid_source()
+        pn[new_key()] = rows[i].tolist()
+    return pn
+
+
+def link_slices(per_slice_rectangles, three_d: bool = False):
+    """link_arrays over per-slice lists of (x_min, y_min, x_max, y_max) rectangles."""
     if not per_slice_rectangles:
         return {}
-    pn = gen_particle_neighborhoods(per_slice_rectangles[0], 1)
-    t = _Table(pn)
-    for idx in range(2, len(per_slice_rectangles) + 1):
-        _update_table(t, per_slice_rectangles[idx - 1], idx, three_d, _uuid_source())
-    t.store()
-    finalize_particle_neighborhoods_(pn)
-    return pn
+    counts = np.fromiter((len(r) for r in per_slice_rectangles), np.int64, len(per_slice_rectangles))
+    offsets = np.zeros(counts.size + 1, np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    flat = np.array([c for r in per_slice_rectangles for br in r for c in br], np.int32).reshape(-1, 4)
+    return link_arrays(flat, offsets, three_d)
 
 
 def delete_duplicates_(particle_neighborhoods) -> None:

@@ -12,6 +12,7 @@ Volumes are tensors of shape (slices, ny, nx): the reference's (nx, ny, slices) 
 """
 from __future__ import annotations
 
+import warnings
 from typing import Callable, Dict, List, Optional, Tuple
 
 import numpy as np
@@ -19,7 +20,7 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .ccl import link_slices
+from .ccl import link_arrays
 
 __all__ = ["cu_dilate", "slice_bounding_rectangles", "particle_bounding_boxes", "particle_bounding_boxes_3d",
            "tamura", "depth_profile", "getcenterfromslice", "gaussian_kernel", "particle_coordinates"]
@@ -45,12 +46,10 @@ def cu_dilate(vol: torch.Tensor, blocksize: int = 32) -> torch.Tensor:
     return out
 
 
-def slice_bounding_rectangles(vol: torch.Tensor, threshold: Optional[float] = None,
-                              max_boxes: int = 4096) -> List[List[Tuple[int, int, int, int]]]:
-    """Per slice, the rectangles (x_min, y_min, x_max, y_max) of its 8-connected components in
-    ascending label order (cu_find_valid_labels + get_bounding_rectangles of each slice).
-    ``vol``: Bool/uint8 (nonzero is foreground), float32 (equal positive values connect), or
-    uint8 N0f8 with ``threshold`` (foreground where v/255 <= threshold)."""
+def _slice_boxes(vol: torch.Tensor, threshold: Optional[float], max_boxes: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Rectangles (x_min, y_min, x_max, y_max) of every slice's 8-connected components in
+    ascending label order, as int32 (R, 4) rows with int64 (slices + 1,) slice offsets. Only the
+    used part of the device box table is copied to the host."""
     if vol.dim() != 3:
         raise AssertionError("the volume must have shape (slices, ny, nx)")
     v = vol.contiguous()
@@ -75,22 +74,38 @@ def slice_bounding_rectangles(vol: torch.Tensor, threshold: Optional[float] = No
         check(lib.holo_ccl_boxes(h.ptr, stream, nx, ny, count, src.data_ptr(), kind,
                                  float(threshold) if threshold is not None else 0.0, max_boxes,
                                  boxes.data_ptr(), nbox.data_ptr()))
-        n = nbox.cpu().numpy()
+        n = nbox.cpu().numpy().astype(np.int64)
         if count == 0 or int(n.max()) <= max_boxes:
             break
         max_boxes = int(n.max())
-    b = boxes.cpu().numpy()
-    out = []
-    for z in range(count):
-        rows = b[z, :n[z]]
-        rows = rows[np.argsort(rows[:, 0].view(np.uint32), kind="stable")]
-        # tensor (row j, column i) is the reference's [y = i + 1, x = j + 1]
-        out.append([(int(r[2]) + 1, int(r[1]) + 1, int(r[4]) + 1, int(r[3]) + 1) for r in rows])
-    return out
+    offsets = np.zeros(count + 1, np.int64)
+    np.cumsum(n, out=offsets[1:])
+    maxn = int(n.max()) if count else 0
+    if maxn == 0:
+        return np.zeros((0, 4), np.int32), offsets
+    b = boxes[:, :maxn].cpu().numpy()
+    valid = np.arange(maxn)[None, :] < n[:, None]
+    key = b[..., 0].view(np.uint32).astype(np.int64)
+    key[~valid] = 1 << 40  # unused slots sort last
+    b = np.take_along_axis(b, np.argsort(key, axis=1, kind="stable")[..., None], axis=1)
+    # box (i0, j0, i1, j1): tensor (row j, column i) is the reference's [y = i + 1, x = j + 1]
+    rects = (b[valid][:, [2, 1, 4, 3]] + 1).astype(np.int32)
+    return rects, offsets
+
+
+def slice_bounding_rectangles(vol: torch.Tensor, threshold: Optional[float] = None,
+                              max_boxes: int = 4096) -> List[List[Tuple[int, int, int, int]]]:
+    """Per slice, the rectangles (x_min, y_min, x_max, y_max) of its 8-connected components in
+    ascending label order (cu_find_valid_labels + get_bounding_rectangles of each slice).
+    ``vol``: Bool/uint8 (nonzero is foreground), float32 (equal positive values connect), or
+    uint8 N0f8 with ``threshold`` (foreground where v/255 <= threshold)."""
+    rects, offsets = _slice_boxes(vol, threshold, max_boxes)
+    return [[tuple(int(c) for c in r) for r in rects[offsets[z]:offsets[z + 1]]] for z in range(offsets.size - 1)]
 
 
 def _bounding_boxes(vol, threshold, three_d):
-    return link_slices(slice_bounding_rectangles(vol, threshold), three_d)
+    rects, offsets = _slice_boxes(vol, threshold, 4096)
+    return link_arrays(rects, offsets, three_d)
 
 
 def particle_bounding_boxes(d_bin_vol: torch.Tensor, threshold: Optional[float] = None):
@@ -114,7 +129,13 @@ def tamura(arr: np.ndarray) -> float:
 
 
 def depth_profile(f: Callable, subvol: np.ndarray) -> np.ndarray:
-    """particle_detection.jl:114-116: ``f`` of every slice of a (slices, rows, columns) block."""
+    """particle_detection.jl:114-116: ``f`` of every slice of a (slices, rows, columns) block.
+    The Tamura metric is evaluated for all slices at once (the same FP64 std / mean)."""
+    if f is tamura:
+        a = np.asarray(subvol, dtype=np.float64).reshape(subvol.shape[0], -1)
+        with np.errstate(invalid="ignore", divide="ignore"), warnings.catch_warnings():
+            warnings.simplefilter("ignore", RuntimeWarning)  # one-pixel slices: std is NaN, as in Julia
+            return a.std(axis=1, ddof=1) / a.mean(axis=1)
     return np.array([f(subvol[k]) for k in range(subvol.shape[0])], dtype=np.float64)
 
 

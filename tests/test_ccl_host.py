@@ -126,15 +126,43 @@ def test_vectorised_linking_equals_scalar_loops(three_d, seed):
     assert list(pn.values()) == ref
 
 
-@pytest.mark.parametrize("three_d", [False, True])
-def test_link_slices_equals_per_call_updates(three_d):
-    rng = np.random.default_rng(7)
-    slices = [[tuple(int(v) for v in (x, y, x + w, y + h))
-               for x, y, w, h in zip(rng.integers(1, 80, k), rng.integers(1, 80, k), rng.integers(2, 9, k),
-                                     rng.integers(2, 9, k))] for k in rng.integers(0, 10, 30)]
+def _per_call(slices, three_d):
     pn = ccl.gen_particle_neighborhoods(slices[0], 1)
     upd = ccl.update_particle_neighborhoods3d_ if three_d else ccl.update_particle_neighborhoods_
     for z in range(2, len(slices) + 1):
         upd(pn, slices[z - 1], z)
     ccl.finalize_particle_neighborhoods_(pn)
-    assert list(ccl.link_slices(slices, three_d).values()) == list(pn.values())
+    return list(pn.values())
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+@pytest.mark.parametrize("seed,nmax,wmin,extent", [(7, 10, 2, 80), (8, 6, 0, 40), (9, 14, 1, 120), (10, 3, 1, 20)])
+def test_native_linking_equals_per_call_updates(three_d, seed, nmax, wmin, extent):
+    """libholo's holo_link_rectangles (link_slices) against the reference's per-call sequence
+    gen / update! per slice / finalize!, itself checked against the scalar loops above; widths
+    from 0 or 1 make unit cubes and degenerate extents."""
+    rng = np.random.default_rng(seed)
+    slices = [[tuple(int(v) for v in (x, y, x + w, y + h))
+               for x, y, w, h in zip(rng.integers(1, extent, k), rng.integers(1, extent, k), rng.integers(wmin, 9, k),
+                                     rng.integers(wmin, 9, k))] for k in rng.integers(0, nmax, 60)]
+    assert list(ccl.link_slices(slices, three_d).values()) == _per_call(slices, three_d)
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+def test_native_linking_edge_cases(three_d):
+    assert ccl.link_slices([], three_d) == {}
+    assert ccl.link_slices([[], [], []], three_d) == {}
+    one = [[(1, 1, 5, 4), (10, 10, 12, 12)]]
+    assert list(ccl.link_slices(one, three_d).values()) == _per_call(one, three_d)
+    tall = [[(3, 3, 7, 7)]] * 5
+    assert list(ccl.link_slices(tall, three_d).values()) == [[3, 3, 1, 7, 7, 5]]
+
+
+def test_native_linking_rejects_bad_offsets():
+    rects = np.zeros((2, 4), np.int32)
+    with pytest.raises(AssertionError, match="decrease"):
+        ccl.link_arrays(rects, np.array([0, 2, 1], np.int64))
+    with pytest.raises(AssertionError, match="offsets"):
+        ccl.link_arrays(rects, np.array([1, 2], np.int64))
+    with pytest.raises(AssertionError, match="fewer rectangles"):
+        ccl.link_arrays(rects, np.array([0, 3], np.int64))

@@ -49,6 +49,14 @@ def test_tamura_and_centre_closed_forms():
     assert pd.getcenterfromslice(two) == (2.5, 2.5)
 
 
+def test_vectorised_tamura_profile_equals_per_slice_loop():
+    rng = np.random.default_rng(4)
+    block = rng.integers(0, 256, size=(37, 6, 9)).astype(np.float32) / np.float32(255)
+    loop = np.array([pd.tamura(block[k]) for k in range(block.shape[0])])
+    np.testing.assert_array_equal(pd.depth_profile(pd.tamura, block), loop)
+    assert np.isnan(pd.depth_profile(pd.tamura, block[:, :1, :1])).all()  # one pixel: std is NaN
+
+
 def test_particle_coordinates_of_a_synthetic_block():
     """A dark spot whose contrast peaks at slice 13 (1-based), centred at tensor (row 6, col 9):
     depth 13 and centre of mass in 1-based reference coordinates."""
