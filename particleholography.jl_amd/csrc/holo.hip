@@ -810,7 +810,9 @@ int holo_create(holo_handle* out, int device) {
   if (cap > 0) h->max_lds = static_cast<int>(std::min<size_t>(cap, h->max_lds));
   h->col_e = static_cast<int>(env_size("HOLO_COL_E", 0));
   // Per-size tiles chosen by holo_autotune on an MI355X and confirmed on single-hologram
-  // reconstructions (768^2 and 1280x1024 +37%, 2048^2 +14%, 512^2 +5% over the generic rule);
+  // reconstructions (768^2 and 1280x1024 +37%, 512^2 +5% over the generic rule). 2048^2 comes
+  // from a direct sweep: no useful round fits the 256 MB Infinity Cache at that size, so long
+  // rounds (64 slices, 2 GB) amortise each launch's tail (53.8k slices/s against 48.9k with 16);
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
@@ -819,7 +821,7 @@ int holo_create(holo_handle* out, int device) {
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
                                        {768, 768, 56, 16, 4, 4, 2},
                                        {1280, 1024, 25, 8, 8, 4, 4},
-                                       {2048, 2048, 16, 16, 4, 4, 2}};
+                                       {2048, 2048, 64, 16, 16, 4, 2}};
     for (const Shipped& e : kShipped) {
       Tuning t;
       t.batch = e.batch;
@@ -1551,8 +1553,11 @@ int holo_reconst_batch(holo_handle h, void* stream, const holo_job* jobs, int nj
     HOLO_TRY(make_geom(h, j0.nx, j0.ny, &g, &Ex, &Ey, 512));
     int maxs = 0;
     for (size_t q = g0; q < g1; ++q) maxs = std::max(maxs, jobs[order[q]].slices);
-    const ReconPlan rp = recon_plan(h, g, maxs);
+    ReconPlan rp = recon_plan(h, g, maxs);
     const size_t plane = static_cast<size_t>(g.nx) * g.ny;
+    // one job's spectrum and round of slices stay within the workspace budget where they can
+    const size_t fit = budget / (plane * sizeof(float2));
+    if (fit >= 2 && static_cast<size_t>(rp.m) > fit - 1) rp.m = static_cast<int>(fit - 1);
     const size_t perjob = plane * sizeof(float2) * (1 + static_cast<size_t>(rp.m));
     // Holograms per launch: as many as keep one round's spectra and column-transformed
     // slices near the 256 MB Infinity Cache, where pass B finds what pass A wrote (measured:
@@ -1646,11 +1651,19 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
   HOLO_TRY(check_launch());
   Tuning best;
   double best_total = 1e30;
-  const size_t mb[2] = {128ull << 20, 256ull << 20};
-  for (size_t bi = 0; bi < 2; ++bi) {
+  // rounds near half and near all of the Infinity Cache (pass B then reads pass A's slices
+  // from it), and 2 GB rounds for sides where a cache-sized round holds fewer slices than a
+  // column workgroup advances (16): there long rounds amortise each launch's tail instead. (At
+  // 1024^2 long rounds can win a single-hologram race yet cost batched calls their
+  // several-holograms-per-launch rounds, 1.06 -> 0.89 of the roofline on the 64-hologram mix.)
+  const size_t mb[3] = {128ull << 20, 256ull << 20, 2048ull << 20};
+  int tried[3] = {0, 0, 0};
+  for (size_t bi = 0; bi < 3; ++bi) {
+    if (bi == 2 && mb[1] / pbytes >= 16) continue;
     Tuning t;
     t.batch = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, mb[bi] / pbytes)));
-    if (bi == 1 && t.batch == best.batch) continue;
+    tried[bi] = t.batch;
+    if ((bi >= 1 && t.batch == tried[0]) || (bi == 2 && t.batch == tried[1])) continue;
     double ba = 1e30, bb = 1e30;
     Tuning ta = t, tb = t;
     for (int W : {2, 4, 8, 16})
