@@ -284,13 +284,13 @@ Tuning eff_tuning(const holo_context* h, int nx, int ny) {
   return t;
 }
 
-// Tiles of the Gerchberg-Saxton passes. A single pair is one plane per launch: at the
-// reconstruction's 16 elements per thread a 1024^2 row pass is one wave per SIMD, so GS runs
-// its lines with fewer elements per thread (more waves, one more LDS exchange) and narrower
-// tiles. Measured on MI355X (E rows / columns, iterations/s): 1024^2 E 4/8 on 4-column and
+// Tiles of passes over a single plane (Gerchberg-Saxton, cu_asm_prop!, the low-pass filter
+// apply, the 2-D FFT). One plane per launch: at the reconstruction's 16 elements per thread a
+// 1024^2 row pass is one wave per SIMD, so these passes run their lines with fewer elements
+// per thread (more waves, one more LDS exchange) and narrower tiles. Measured on MI355X (E rows / columns, iterations/s): 1024^2 E 4/8 on 4-column and
 // 1-row tiles 20.9k (16.2k at the reconstruction's E 16/16, 20.3k at E 8/8); 512^2 E 4/8, 4 x 1
 // 42.0k; 2048^2 E 8/16, 4 x 1 8.7k. Global tuning (holo_set_tuning) still sets the tile sizes.
-Tuning gs_tiles(const holo_context* h, int nx, int ny) {
+Tuning plane_tiles(const holo_context* h, int nx, int ny) {
   Tuning t;
   if (nx == 1024 && ny == 1024) {
     t.erow = 4;
@@ -924,7 +924,8 @@ int holo_fft2(holo_handle h, void* stream, int nx, int ny, const void* in, void*
   if (!h || !in || !out) return fail(HOLO_ERR_ARG, "NULL argument");
   Geom g;
   int Ex, Ey;
-  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  const Tuning pt = plane_tiles(h, nx, ny);
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &pt));
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int dir = inverse ? +1 : -1;
   const float scale = inverse ? static_cast<float>(1.0 / (static_cast<double>(nx) * ny)) : 1.0f;
@@ -939,7 +940,8 @@ int holo_asm_prop(holo_handle h, void* stream, int nx, int ny, void* out, const 
   HOLO_TRY(validate_transfer(tf, "tf"));
   Geom g;
   int Ex, Ey;
-  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  const Tuning pt = plane_tiles(h, nx, ny);
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &pt));
   const size_t plane = static_cast<size_t>(nx) * ny;
   HOLO_TRY(ensure_ws(h, plane * sizeof(float2)));
   float2* B = static_cast<float2*>(h->ws);
@@ -978,7 +980,8 @@ int holo_apply_low_pass_filter(holo_handle h, void* stream, int nx, int ny, void
   if (!h || !in || !out || !lpf) return fail(HOLO_ERR_ARG, "NULL argument");
   Geom g;
   int Ex, Ey;
-  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey));
+  const Tuning pt = plane_tiles(h, nx, ny);
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &pt));
   const size_t plane = static_cast<size_t>(nx) * ny;
   HOLO_TRY(ensure_ws(h, plane * sizeof(float2)));
   float2* B = static_cast<float2*>(h->ws);
@@ -1375,7 +1378,7 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
   if (iters == 0) return holo_gabor_wavefront(h, stream, nx, ny, holo1, out);
   Geom g;
   int Ex, Ey;
-  const Tuning gt = gs_tiles(h, nx, ny);
+  const Tuning gt = plane_tiles(h, nx, ny);
   HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &gt));
   const size_t plane = static_cast<size_t>(nx) * ny;
   // workspace: [field | H fwd | H bwd | I1 | I2]
@@ -1437,7 +1440,7 @@ int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* j
     const int nx = jobs[order[g0]].nx, ny = jobs[order[g0]].ny;
     Geom g;
     int Ex, Ey;
-    const Tuning gt = gs_tiles(h, nx, ny);
+    const Tuning gt = plane_tiles(h, nx, ny);
     HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &gt));
     const size_t plane = static_cast<size_t>(nx) * ny;
     const size_t perpair = 3 * plane * sizeof(float2);
