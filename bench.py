@@ -392,7 +392,7 @@ def bench_serving(torch, nreq):
     from particleholography import serve
 
     reqs = serve.make_stream(nreq, rate=0.0, seed=105)
-    srv = serve.Server(n_slots=4, max_batch=16, hbm_budget=64 << 30, workspace_per_slot=4 << 30)
+    srv = serve.Server(n_slots=2, max_batch=16, hbm_budget=64 << 30, workspace_per_slot=4 << 30)
     srv.run(reqs[: min(32, nreq)])  # warm-up: plans, workspaces
     cap = srv.run(reqs)
     rate = 0.75 * cap["requests"] / cap["makespan_s"]

@@ -114,14 +114,19 @@ def select_batch(waiting: deque, used: int, budget: int, max_batch: int, others_
 
 
 # Round sizes of the slot handles where they differ from a lone stream's. A lone 2048^2
-# reconstruction runs 64-slice rounds to amortise each launch's tail; with four slots in flight
-# the other slots fill those tails, and 16-slice rounds serve more (capacity of the 1000-request
-# stream on MI355X: 171k 1024^2-equivalent slices/s against 159k with 64-slice rounds).
+# reconstruction runs 64-slice rounds to amortise each launch's tail; with other slots in flight
+# they fill those tails, and 16-slice rounds serve more (capacity of the 1000-request stream on
+# MI355X, 1024^2-equivalent slices/s: four slots 171k against 159k with 64-slice rounds, two
+# slots 176k against 172k).
 SLOT_TUNING = {(2048, 2048): (16, 16, 4, 4, 2)}  # batch, spg_cols, spg_rows, cols, rows
 
 
 class Server:
-    def __init__(self, device: int = 0, n_slots: int = 4, max_batch: int = 32,
+    # Two slots: measured on the 1000-request stream (capacity, then p50/p99 at 75% of it),
+    # four slots 165k eq. slices/s, 12.2 / 132 ms; three 171k, 13.8 / 116 ms; two 176k,
+    # 15.2 / 113 ms. Fewer slots share the device among fewer groups, so the 2048^2 requests
+    # that make the tail finish sooner; small requests wait a little longer.
+    def __init__(self, device: int = 0, n_slots: int = 2, max_batch: int = 32,
                  hbm_budget: int = 32 << 30, workspace_per_slot: int = 2 << 30,
                  max_group_work: Optional[int] = 32 << 30):
         self.device = device
