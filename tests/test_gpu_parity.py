@@ -36,7 +36,8 @@ def _t(a, dev):
 # ---------------------------------------------------------------------------------------
 FFT_SIZES = [(1, 1), (2, 2), (4, 4), (8, 8), (16, 16), (32, 32), (64, 64), (3, 3), (5, 5), (6, 6),
              (12, 12), (20, 20), (24, 24), (40, 40), (48, 48), (96, 80), (128, 256), (384, 384),
-             (640, 640), (512, 512), (768, 768), (1024, 1024), (1280, 1024), (2048, 2048), (4096, 64)]
+             (640, 640), (512, 512), (768, 768), (1024, 1024), (1280, 1024), (2048, 2048), (4096, 64),
+             (64, 4096), (4096, 4096)]
 
 
 @pytest.mark.parametrize("nx,ny", FFT_SIZES)
@@ -102,6 +103,27 @@ def test_projection_generated_vs_fp64_oracle(gpu, n, slices, z0, dz):
     # zero imaginary part
     fused = ph.reconst_xyprojection_holo(_t(h, gpu), h0, hdz, slices).cpu().numpy()
     assert asm.rel_l2(fused, got) < 1e-6
+
+
+@pytest.mark.parametrize("nx,ny,slices", [(512, 512, 40), (768, 768, 60), (1024, 1024, 70), (1280, 1024, 30),
+                                          (2048, 2048, 66), (4096, 4096, 3)])
+def test_projection_configured_sizes_vs_fp64_oracle(gpu, nx, ny, slices):
+    """The production kernels and tiles at the configured sides (compile-time plans, shipped
+    per-size rounds: 32 slices at 1024^2, 25 at 1280x1024, 64 at 2048^2, so the slice counts
+    cross round boundaries) and at the largest supported side, against the FP64 oracle."""
+    from particleholography.batch import ReconJob, reconst_batch
+
+    ph = _ph()
+    z0, dz = -1e4, -300.0
+    h = _holo(nx, nx + slices, ny=ny)
+    want = asm.reconst_xyprojection(asm.gabor_wavefront(h), slices, "exact", z_front=z0, z_dz=dz, wavlen=LAM, dx=DX)
+    got = reconst_batch([ReconJob(_t(h, gpu), z0, dz, slices, LAM, DX)])[0].cpu().numpy()
+    assert asm.rel_l2(got, want) < 1e-4
+    if nx == ny:
+        dsq = ph.cu_transfer_sqrt_arr(nx, LAM, DX)
+        h0, hdz = ph.cu_transfer(z0, nx, LAM, dsq), ph.cu_transfer(dz, nx, LAM, dsq)
+        single = ph.cu_get_reconst_xyprojection(ph.cu_gabor_wavefront(_t(h, gpu)), h0, hdz, slices).cpu().numpy()
+        assert asm.rel_l2(single, want) < 1e-4
 
 
 @pytest.mark.parametrize("batch,spgc,spgr", [(1, 1, 1), (3, 2, 1), (4, 3, 5), (64, 7, 2)])
@@ -270,8 +292,10 @@ def _gs_case(n, seed, z=2000.0):
     return i1, i2
 
 
-@pytest.mark.parametrize("n,iters", [(64, 1), (64, 10), (128, 25)])
+@pytest.mark.parametrize("n,iters", [(64, 1), (64, 10), (128, 25), (512, 20), (1024, 20), (2048, 3)])
 def test_gs_generated_vs_fp64_oracle(gpu, n, iters):
+    """Small sides on runtime plans; 512^2, 1024^2 and 2048^2 on the single-plane line shapes
+    and their compile-time plans (20 iterations: one 16-iteration graph plus a remainder)."""
     ph = _ph()
     z = 2000.0
     i1, i2 = _gs_case(n, n + iters, z)
@@ -353,7 +377,7 @@ def test_asm_prop_reference_case(gpu):
     np.testing.assert_allclose(o, asm.asm_prop(np.ones((4, 4), complex), 100.0, LAM, DX), atol=1e-6)
 
 
-@pytest.mark.parametrize("n,z", [(64, 5000.0), (256, -3e4), (768, 1e4)])
+@pytest.mark.parametrize("n,z", [(64, 5000.0), (256, -3e4), (768, 1e4), (512, 2000.0), (1024, -8e4)])
 def test_asm_prop_vs_oracle(gpu, n, z):
     ph = _ph()
     rng = np.random.default_rng(n)
