@@ -244,6 +244,32 @@ def test_complex_volume(gpu):
         assert asm.rel_l2(cv[k], want[k]) < 1e-5
 
 
+@pytest.mark.parametrize("n,slices,rt", [(1024, 40, "n0f8"), (2048, 8, "n0f8"), (768, 60, "f32"), (1024, 4, "c64")])
+def test_volumes_configured_sizes_vs_fp64_oracle(gpu, n, slices, rt):
+    """The store epilogues on the production kernels and tiles (1024^2 crosses a 32-slice
+    round): N0f8 within 1 LSB, Float32 within 2e-5, ComplexF32 within 1e-5 rel L2 per slice."""
+    ph = _ph()
+    h = (0.5 * _holo(n, n + slices, amp=0.6)).astype(np.float32)  # straddles the clamp at 1
+    dsq = ph.cu_transfer_sqrt_arr(n, LAM, DX)
+    z0, dz = -8e3, -700.0
+    h0, hdz = ph.cu_transfer(z0, n, LAM, dsq), ph.cu_transfer(dz, n, LAM, dsq)
+    wf = ph.cu_gabor_wavefront(_t(h, gpu))
+    kw = dict(z_front=z0, z_dz=dz, wavlen=LAM, dx=DX)
+    if rt == "c64":
+        cv = ph.cu_get_reconst_complex_vol(wf, h0, hdz, slices).cpu().numpy()
+        want = asm.reconst_complex_vol(asm.gabor_wavefront(h), slices, "exact", **kw)
+        assert max(asm.rel_l2(cv[k], want[k]) for k in range(slices)) < 1e-5
+    elif rt == "n0f8":
+        vol = ph.cu_get_reconst_vol(wf, h0, hdz, slices).cpu().numpy()
+        want = asm.reconst_vol(asm.gabor_wavefront(h), slices, np.uint8, "exact", **kw)
+        diff = np.abs(vol.astype(int) - want.astype(int))
+        assert diff.max() <= 1 and np.mean(diff) < 0.01
+    else:
+        vol = ph.cu_get_reconst_vol(wf, h0, hdz, slices, return_type=ph.Float32).cpu().numpy()
+        want = asm.reconst_vol(asm.gabor_wavefront(h), slices, np.float32, "exact", **kw)
+        assert np.max(np.abs(vol.astype(np.float64) - want.astype(np.float64))) < 2e-5
+
+
 @pytest.mark.parametrize("rt", ["n0f8", "f32"])
 def test_volume_and_projection(gpu, rt):
     ph, h, wf, h0, hdz, kw = _vol_case(gpu)
