@@ -44,8 +44,9 @@ def reconst_xyprojection_cpu(holo: np.ndarray, h0: np.ndarray, hdz: np.ndarray, 
 def measure(holo: np.ndarray, z0: float, dz: float, wavlen: float, dx: float, slices: int,
             budget_s: float = 15.0, threads: int | None = None) -> dict:
     """Times the CPU restatement on ``threads`` host threads (default: every core in the
-    affinity mask). The sample is bounded: the slice count is cut so one run takes about
-    ``budget_s / 3`` seconds; returns slices/s over the median of 3 runs."""
+    affinity mask) over a bounded sample of about ``budget_s`` seconds of work: one run covers
+    the whole workload, or as many slices as fit a third of the budget, and runs repeat (3 to
+    25) until the budget is spent; returns slices/s over the median run."""
     import torch
 
     n_threads = threads or cores()
@@ -54,16 +55,20 @@ def measure(holo: np.ndarray, z0: float, dz: float, wavlen: float, dx: float, sl
     d32 = asm.transfer_sqrt_arr(nx, wavlen, dx, ny)
     h0 = asm.transfer_reference(z0, wavlen, d32)
     hdz = asm.transfer_reference(dz, wavlen, d32)
+    reconst_xyprojection_cpu(holo, h0, hdz, 2)  # warm-up (first-call costs)
+    calib = min(slices, 16)
     t = time.perf_counter()
-    reconst_xyprojection_cpu(holo, h0, hdz, 2)  # warm-up, also sizes the sample
-    per_slice = max((time.perf_counter() - t) / 2.0, 1e-4)
+    reconst_xyprojection_cpu(holo, h0, hdz, calib)
+    per_slice = max((time.perf_counter() - t) / calib, 1e-5)
     sample = int(max(2, min(slices, (budget_s / 3.0) / per_slice)))
+    runs = int(min(25, max(3, round(budget_s / (sample * per_slice)))))
     times = []
-    for _ in range(3):
+    for _ in range(runs):
         t = time.perf_counter()
         reconst_xyprojection_cpu(holo, h0, hdz, sample)
         times.append(time.perf_counter() - t)
     med = float(np.median(times))
     return {"value": sample / med, "unit": "z-slices/s", "cores": n_threads, "kind": "port",
-            "sample": f"{nx}x{ny} hologram, {sample} of {slices} slices, median of 3 runs, "
-                      f"torch {torch.__version__} CPU complex64 FFT on {n_threads} threads"}
+            "sample": f"{nx}x{ny} hologram, {sample} of {slices} slices, median of {runs} runs "
+                      f"({sum(times):.1f} s of CPU work), torch {torch.__version__} CPU complex64 FFT "
+                      f"on {n_threads} threads"}
