@@ -61,12 +61,14 @@ def measure(holo: np.ndarray, z0: float, dz: float, wavlen: float, dx: float, sl
     reconst_xyprojection_cpu(holo, h0, hdz, calib)
     per_slice = max((time.perf_counter() - t) / calib, 1e-5)
     sample = int(max(2, min(slices, (budget_s / 3.0) / per_slice)))
-    runs = int(min(25, max(3, round(budget_s / (sample * per_slice)))))
     times = []
-    for _ in range(runs):
+    runs = 3
+    while len(times) < runs:
         t = time.perf_counter()
         reconst_xyprojection_cpu(holo, h0, hdz, sample)
         times.append(time.perf_counter() - t)
+        if len(times) == 1:  # the first full run sizes the repetition count
+            runs = int(min(25, max(3, round(budget_s / times[0]))))
     med = float(np.median(times))
     return {"value": sample / med, "unit": "z-slices/s", "cores": n_threads, "kind": "port",
             "sample": f"{nx}x{ny} hologram, {sample} of {slices} slices, median of {runs} runs "
