@@ -147,3 +147,19 @@ def test_gs_batch_many_pairs_vs_oracle(gpu):
     for (i1, i2), out in zip(pairs, outs):
         ref = asm.phase_retrieval(i1, i2, 3, z=2000.0, wavlen=LAM, dx=DX)
         assert asm.weighted_phase_rms(out.cpu().numpy(), ref) < 1e-3
+
+
+@pytest.mark.parametrize("nx,ny,iters", [(96, 64, 4), (1280, 1024, 3)])
+def test_gs_batch_non_square_vs_oracle(gpu, nx, ny, iters):
+    """The batched entry point takes (nx, ny) pairs (the reference's GS is square-only)."""
+    from particleholography.batch import GSJob, phase_retrieval_batch
+
+    rng = np.random.default_rng(nx + ny)
+    psi = np.exp(1j * 0.7 * rng.standard_normal((ny, nx))) * (1 + 0.3 * rng.random((ny, nx)))
+    i1 = (np.abs(psi) ** 2).astype(np.float32)
+    i2 = (np.abs(asm.asm_prop(psi, 2000.0, LAM, DX)) ** 2).astype(np.float32)
+    out = phase_retrieval_batch([GSJob(torch.from_numpy(i1).to(gpu), torch.from_numpy(i2).to(gpu), 2000.0, iters)])[0]
+    assert tuple(out.shape) == (ny, nx)
+    ref = asm.phase_retrieval(i1, i2, iters, z=2000.0, wavlen=LAM, dx=DX)
+    np.testing.assert_allclose(np.abs(out.cpu().numpy()), np.sqrt(i1), rtol=2e-6)
+    assert asm.weighted_phase_rms(out.cpu().numpy(), ref) < 1e-3
