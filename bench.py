@@ -181,6 +181,7 @@ def main():
         extras["plain_path_slices_per_s"] = round(plain, 1)
         extras["speedup_vs_plain"] = round(slices_per_s / plain, 2)
         extras["gs_1024"] = bench_gs(ph, synth, torch, dev, d_sqr)
+        extras["volumes_1024"] = bench_volumes(ph, torch, holo)
         extras["lpf_1024"] = bench_lpf(ph, torch, holo)
         extras["ingest_1024"] = bench_ingest(ph, torch, dev)
         extras["detect_reference_holograms"] = bench_detect(ph, torch, dev)
@@ -241,6 +242,29 @@ def bench_gs(ph, synth, torch, dev, d_sqr):
     tw, _ = timed_steps(run, 3, 1, torch)
     return {"iterations_per_s": round(300 / gw, 1), "gs_algorithmic_GBs": round(72.0 * N * N * 300 / gw / 1e9, 1),
             "pipeline_ms": round(tw / 3 * 1e3, 3), "config": "1024^2, 100 iterations at 2000 um, then 512 slices"}
+
+
+def bench_volumes(ph, torch, holo, reps=3):
+    """The volume variants on the headline hologram (1024^2, dz = 200 um from z0 = 1e4 um):
+    cu_get_reconst_vol N0f8 and Float32 and cu_get_reconst_vol_and_xyprojection N0f8 over 1000
+    slices, cu_get_reconst_complex_vol over 250 slices (2 GiB). Algorithmic bytes per pixel and
+    slice: pass A 16, plus the column-transformed slice read 8 and the stored value (1 / 4 / 8),
+    plus 8 for the projection's read-modify-write."""
+    d_sqr = ph.cu_transfer_sqrt_arr(N, LAM, DX)
+    h0, hdz = ph.cu_transfer(-Z0, N, LAM, d_sqr), ph.cu_transfer(-DZ, N, LAM, d_sqr)
+    wf = ph.cu_gabor_wavefront(holo)
+    cases = {"n0f8": (SLICES, 25, lambda s: ph.cu_get_reconst_vol(wf, h0, hdz, s)),
+             "f32": (SLICES, 28, lambda s: ph.cu_get_reconst_vol(wf, h0, hdz, s, return_type=ph.Float32)),
+             "n0f8_and_projection": (SLICES, 33, lambda s: ph.cu_get_reconst_vol_and_xyprojection(wf, h0, hdz, s)),
+             "complex": (250, 32, lambda s: ph.cu_get_reconst_complex_vol(wf, h0, hdz, s))}
+    out = {}
+    for name, (s, bpx, fn) in cases.items():
+        _, sec = timed_steps(lambda: fn(s), reps, 1, torch)
+        rate = s * reps / sec
+        out[name] = {"slices": s, "slices_per_s": round(rate, 1),
+                     "algorithmic_GBs": round(rate * N * N * bpx / 1e9, 1), "bytes_per_px_slice": bpx}
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_lpf(ph, torch, holo):

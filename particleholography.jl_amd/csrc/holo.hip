@@ -493,6 +493,8 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     return fail(HOLO_ERR_ARG, "generated transfers must share wavlen and dx");
   if (slices < 1) return fail(HOLO_ERR_ARG, "slices must be >= 1");
   if (!src) return fail(HOLO_ERR_ARG, "input is NULL");
+  if ((ox != 0 || oy != 0 || onx != nx || ony != ny) && (kind != 1 || out_dtype != HOLO_N0F8))
+    return fail(HOLO_ERR_ARG, "a cropped output window is an N0f8 volume");
   Geom g;
   int Ex, Ey;
   HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 512));
@@ -582,12 +584,18 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
           allow_lds(kern, rlds);
           hipLaunchKernelGGL(kern, row_grid(g, ry), row_block(g), rlds, st, sa, g);
         };
-        switch (out_dtype) {
-          case HOLO_N0F8: launch(k_recon_rows_store<E, L, 0>); break;
-          case HOLO_F16: launch(k_recon_rows_store<E, L, 1>); break;
-          case HOLO_F32: launch(k_recon_rows_store<E, L, 2>); break;
-          case HOLO_F64: launch(k_recon_rows_store<E, L, 3>); break;
-          default: launch(k_recon_rows_store<E, L, 4>); break;
+        // a cropped window only comes from the padded reconstruction, whose volume is N0f8
+        const bool crop = ox != 0 || oy != 0 || onx != nx || ony != ny;
+        if (crop) {
+          launch(k_recon_rows_store<E, L, 0, true>);
+        } else {
+          switch (out_dtype) {
+            case HOLO_N0F8: launch(k_recon_rows_store<E, L, 0, false>); break;
+            case HOLO_F16: launch(k_recon_rows_store<E, L, 1, false>); break;
+            case HOLO_F32: launch(k_recon_rows_store<E, L, 2, false>); break;
+            case HOLO_F64: launch(k_recon_rows_store<E, L, 3, false>); break;
+            default: launch(k_recon_rows_store<E, L, 4, false>); break;
+          }
         }
       }
     });

@@ -462,7 +462,25 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
 // Store epilogue (A7 volume, A8 complex volume, A9 projection of the stored values, and
 // the padded variant's centre window). OUT: 0 N0f8, 1 Float16, 2 Float32, 3 Float64,
 // 4 ComplexF32. Intensities are clamp(|psi|^2, 0, 1) (holofunc.jl:158-166); N0f8 is
-// round(UInt8, v*255) with ties to even, computed in FP64 like the reference's promotion.
+// round(UInt8, c*255) for c in [0, 1] with ties to even, as the reference computes it in FP64
+// (holofunc.jl:152-154): the FP64 product of a float and 255 is exact. Here in FP32: p = c*255
+// rounded; a half-integer is a float, so an exact product never rounds across one, and only a
+// product that rounds onto k + 1/2 can round differently from the exact one; the exact FMA
+// residual then tells on which side the exact product lies. Equal to the FP64 rule for every
+// float in [0, 1] (checked exhaustively on the host).
+__device__ __forceinline__ unsigned n0f8_round(float c) {
+#pragma clang fp contract(off)
+  const float p = c * 255.0f;
+  float r = rintf(p);
+  const float fl = floorf(p);
+  if (p - fl == 0.5f) {
+    const float e = fmaf(c, 255.0f, -p);
+    if (e > 0.0f) r = fl + 1.0f;
+    else if (e < 0.0f) r = fl;
+  }
+  return static_cast<unsigned>(r);
+}
+
 // A NaN intensity is stored as 0 in N0f8 (the reference raises InexactError there).
 struct StoreArgs {
   const float2* G;
@@ -486,10 +504,11 @@ __device__ __forceinline__ float store_value(void* vol, size_t idx, float2 psi) 
     const float in = psi.x * psi.x + psi.y * psi.y;
     const float cl = in != in ? in : fminf(fmaxf(in, 0.f), 1.f);
     if constexpr (OUT == 0) {
-      const double r = rint(static_cast<double>(cl) * 255.0);
-      const unsigned char u = cl != cl ? 0 : static_cast<unsigned char>(r);
-      static_cast<unsigned char*>(vol)[idx] = u;
-      return static_cast<float>(u) / 255.0f;
+      // returns the stored byte: k/255 is monotonic in k, so the projection takes the min of
+      // the bytes and divides once (k_recon_rows_store)
+      const unsigned u = cl != cl ? 0u : n0f8_round(cl);
+      static_cast<unsigned char*>(vol)[idx] = static_cast<unsigned char>(u);
+      return static_cast<float>(u);
     } else if constexpr (OUT == 1) {
       const __half h = __float2half_rn(cl);
       static_cast<__half*>(vol)[idx] = h;
@@ -504,14 +523,18 @@ __device__ __forceinline__ float store_value(void* vol, size_t idx, float2 psi) 
   }
 }
 
-template <int E, int L, int OUT>
+// CROP: the output is a window of the transform (the padded reconstruction); without it the
+// window is the whole plane and every element is stored at a constant offset from its row,
+// which keeps the kernel at 4 waves per SIMD (per-element window offsets and checks cost it one
+// workgroup per CU: 153 VGPRs).
+template <int E, int L, int OUT, bool CROP>
 __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
   extern __shared__ float2 lds[];
   const RowCtx c = row_ctx<E, L>(g, lds);
   const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
-  const int oj = c.j - a.oy;
-  const bool rowin = c.ok && oj >= 0 && oj < a.ony;
+  const int oj = CROP ? c.j - a.oy : c.j;
+  const bool rowin = CROP ? (c.ok && oj >= 0 && oj < a.ony) : c.ok;
   const int k0 = blockIdx.y * a.spg;
   const int k1 = min(k0 + a.spg, a.nbatch);
   float mn[E];
@@ -527,16 +550,22 @@ __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
       const size_t sbase = static_cast<size_t>(a.slice0 + k) * a.vstride + static_cast<size_t>(oj) * a.onx;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const int oi = c.t + T * e - a.ox;
-        if (oi >= 0 && oi < a.onx) mn[e] = fminf(mn[e], store_value<OUT>(a.vol, sbase + oi, cscale(v[e], a.sc)));
+        if constexpr (CROP) {
+          const int oi = c.t + T * e - a.ox;
+          if (oi >= 0 && oi < a.onx) mn[e] = fminf(mn[e], store_value<OUT>(a.vol, sbase + oi, cscale(v[e], a.sc)));
+        } else {
+          mn[e] = fminf(mn[e], store_value<OUT>(a.vol, sbase + c.t + T * e, cscale(v[e], a.sc)));
+        }
       }
     }
   }
   if (OUT != 4 && a.proj != nullptr && rowin && k1 > k0) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int oi = c.t + T * e - a.ox;
-      if (oi >= 0 && oi < a.onx) atomic_min_nonneg(a.proj + static_cast<size_t>(oj) * a.onx + oi, mn[e]);
+      const int oi = CROP ? c.t + T * e - a.ox : c.t + T * e;
+      // N0f8: mn holds the smallest stored byte; its value is byte / 255
+      const float pv = OUT == 0 ? mn[e] / 255.0f : mn[e];
+      if (!CROP || (oi >= 0 && oi < a.onx)) atomic_min_nonneg(a.proj + static_cast<size_t>(oj) * a.onx + oi, pv);
     }
   }
 }
