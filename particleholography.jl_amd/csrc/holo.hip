@@ -1261,8 +1261,10 @@ int holo_link_rectangles(const int32_t* rects, const int64_t* offsets, int nslic
   for (int64_t q = 0; q < offsets[1]; ++q) link_start(t, grid, rects + 4 * q, 1);
   t.rebuild_live();
   std::vector<uint32_t> sub, touched;
+  std::vector<int32_t> seen;
+  for (uint32_t i = 0; i < t.size(); ++i) sub.push_back(i);  // slice 1's rows end on slice 1
   for (int s = 2; s <= nslices; ++s)
-    link_update(t, grid, rects + 4 * offsets[s - 1], offsets[s] - offsets[s - 1], s, three_d != 0, sub, touched);
+    link_update(t, grid, rects + 4 * offsets[s - 1], offsets[s] - offsets[s - 1], s, three_d != 0, sub, touched, seen);
   link_finalize(t);
   for (size_t i = 0; i < t.size(); ++i) {
     int64_t* o = rows + 6 * i;

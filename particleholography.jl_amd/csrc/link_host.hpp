@@ -116,14 +116,23 @@ inline void link_start(LinkTable& t, LinkGrid* grid, const int32_t* r, int32_t s
   if (grid) grid->add(i, r[0], r[1], r[2], r[3], nullptr);
 }
 
+// sub: on entry (3-D) the rows grown or started in slice s - 1, possibly repeated; a 3-D row
+// ends on s - 1 exactly when it was grown or started there, so the live ones among them are the
+// candidates. On exit: the rows grown or started in slice s. t.live is kept only without a grid
+// (the grid path does not read it; finalize rebuilds it).
 inline void link_update(LinkTable& t, LinkGrid* grid, const int32_t* rects, int64_t n, int32_t s, bool three_d,
-                        std::vector<uint32_t>& sub, std::vector<uint32_t>& touched) {
-  const size_t m0 = t.size();  // rows started before this slice; t.live lists the live ones
+                        std::vector<uint32_t>& sub, std::vector<uint32_t>& touched, std::vector<int32_t>& seen) {
+  const size_t m0 = t.size();  // rows started before this slice
   touched.clear();
   if (three_d) {
-    sub.clear();
-    for (uint32_t i : t.live)
-      if (t.z1[i] == s - 1) sub.push_back(i);
+    seen.resize(t.size(), 0);
+    size_t w = 0;
+    for (uint32_t i : sub)
+      if (!t.dead[i] && seen[i] != s && t.z1[i] == s - 1) {
+        seen[i] = s;
+        sub[w++] = i;
+      }
+    sub.resize(w);
   }
   for (int64_t q = 0; q < n; ++q) {
     const int32_t* r = rects + 4 * q;
@@ -179,6 +188,11 @@ inline void link_update(LinkTable& t, LinkGrid* grid, const int32_t* rects, int6
   };
   for (uint32_t i : touched) unit(i);
   for (size_t i = m0; i < t.size(); ++i) unit(static_cast<uint32_t>(i));
+  if (three_d) {  // the next slice's candidates
+    sub.assign(touched.begin(), touched.end());
+    for (size_t i = m0; i < t.size(); ++i) sub.push_back(static_cast<uint32_t>(i));
+  }
+  if (grid) return;
   if (died) {
     size_t w = 0;
     for (uint32_t i : t.live)

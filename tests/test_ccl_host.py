@@ -166,3 +166,18 @@ def test_native_linking_rejects_bad_offsets():
         ccl.link_arrays(rects, np.array([1, 2], np.int64))
     with pytest.raises(AssertionError, match="fewer rectangles"):
         ccl.link_arrays(rects, np.array([0, 3], np.int64))
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+def test_native_linking_on_the_reference_holograms_stream(three_d):
+    """The 9634 per-slice rectangles of the reference's test holograms (GS, 1000 N0f8 slices,
+    threshold 10/255; tests/golden/rects_reference_holograms.npz): native linking equals the
+    per-call sequence."""
+    from pathlib import Path
+
+    d = np.load(Path(__file__).resolve().parent / "golden" / "rects_reference_holograms.npz")
+    rects, off = d["rects"], d["off"]
+    slices = [[tuple(int(v) for v in r) for r in rects[off[z]:off[z + 1]]] for z in range(off.size - 1)]
+    got = list(ccl.link_arrays(rects, off, three_d).values())
+    assert got == _per_call(slices, three_d)
+    assert len(got) == (36 if three_d else 23)
