@@ -181,3 +181,32 @@ def test_native_linking_on_the_reference_holograms_stream(three_d):
     got = list(ccl.link_arrays(rects, off, three_d).values())
     assert got == _per_call(slices, three_d)
     assert len(got) == (36 if three_d else 23)
+
+
+from hypothesis import given, settings, strategies as st  # noqa: E402
+
+_coord = st.integers(min_value=-3, max_value=25)
+_rect = st.tuples(_coord, _coord, st.integers(min_value=-2, max_value=7), st.integers(min_value=-2, max_value=7)).map(
+    lambda t: (t[0], t[1], t[0] + t[2], t[1] + t[3]))  # negative widths: inverted rectangles
+_stream = st.lists(st.lists(_rect, max_size=6), min_size=1, max_size=9)
+
+
+@settings(max_examples=300, deadline=None)
+@given(_stream, st.booleans())
+def test_native_linking_property(slices, three_d):
+    """Random streams, degenerate and inverted rectangles included (the latter take the scan
+    without the grid): native linking equals the per-call sequence."""
+    assert list(ccl.link_slices(slices, three_d).values()) == _per_call(slices, three_d)
+
+
+_gcoord = st.integers(min_value=0, max_value=120)
+_grect = st.tuples(_gcoord, _gcoord, st.integers(min_value=0, max_value=40), st.integers(min_value=0, max_value=40)).map(
+    lambda t: (t[0], t[1], t[0] + t[2], t[1] + t[3]))
+
+
+@settings(max_examples=300, deadline=None)
+@given(st.lists(st.lists(_grect, max_size=8), min_size=1, max_size=12), st.booleans())
+def test_native_linking_property_grid(slices, three_d):
+    """Non-inverted rectangles over several grid cells (the 2-D update and the finalize take
+    their candidates from the grid): native linking equals the per-call sequence."""
+    assert list(ccl.link_slices(slices, three_d).values()) == _per_call(slices, three_d)
