@@ -1,8 +1,8 @@
 /* holo.h -- C ABI of the MI355X angular-spectrum / Gerchberg-Saxton library (libholo.so).
  *
  * Every entry point replaces one function of ParticleHolography.jl (src/holofunc.jl,
- * src/frequency_filters.jl, src/utils.jl) or says that it has no counterpart there; the
- * comment above each declaration names it (file:line). All pointers are device pointers
+ * src/frequency_filters.jl, src/utils.jl, src/ccl.jl, src/particle_detection.jl) or says that
+ * it has no counterpart there; the comment above each declaration names it (file:line). All pointers are device pointers
  * (HIP global memory) unless stated; the caller allocates every input and output. Calls are
  * asynchronous and ordered on the given `stream` (a hipStream_t, NULL = the null stream).
  *
@@ -132,7 +132,9 @@ int holo_reconst_vol_and_xyprojection_padded(holo_handle h, void* stream, int nx
 /* ---- A10: cu_phase_retrieval_holo (holofunc.jl:100-129) --------------------------------- */
 /* Gerchberg-Saxton: psi1 = sqrt(I1); `iters` times: psi2 = P_fwd psi1, psi2 <- sqrt(I2) psi2/|psi2|,
  * psi1 = P_bwd psi2, psi1 <- sqrt(I1) psi1/|psi1|. out (ComplexF32 nx*ny) = psi1. With
- * use_graph != 0 one iteration is captured as a hipGraph and replayed. */
+ * use_graph != 0 the identical iterations run as hipGraphs of up to 16 iterations, captured
+ * once per size and tile choice and cached on the handle (they reference only its workspace,
+ * into which I1 and I2 are copied). */
 int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const float* holo1,
                          const float* holo2, const holo_tf* fwd, const holo_tf* bwd,
                          int iters, void* out, int use_graph);
@@ -260,6 +262,10 @@ int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* j
 /* Caps the workspace the batched calls plan for (holograms are processed in chunks that fit;
  * a single hologram larger than the cap still runs). 0 = HOLO_BATCH_WS_BYTES or 8 GiB. */
 int holo_set_workspace_limit(holo_handle h, size_t bytes);
+/* The device workspace the handle holds now (bytes): it grows on demand to the largest call's
+ * need (spectrum, a round of column-transformed slices, GS tables) and is freed by
+ * holo_destroy. No counterpart in the reference (CUDA.jl's pool owns its temporaries). */
+int holo_workspace_size(holo_handle h, size_t* bytes);
 /* Per-size tile choices (0 = default): slices per batch, slices per column / row workgroup,
  * columns per workgroup, rows per workgroup. The global holo_set_tuning overrides them. */
 int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_cols, int spg_rows,

@@ -1735,6 +1735,12 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
   return HOLO_OK;
 }
 
+int holo_workspace_size(holo_handle h, size_t* bytes) {
+  if (!h || !bytes) return fail(HOLO_ERR_ARG, "NULL argument");
+  *bytes = h->ws_bytes;
+  return HOLO_OK;
+}
+
 int holo_set_workspace_limit(holo_handle h, size_t bytes) {
   if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
   h->batch_ws_limit = bytes;

@@ -85,6 +85,7 @@ SIGNATURES = {
     "holo_phase_retrieval_batch": [_vp, _vp, ctypes.POINTER(holo_gs_job), _i],
     "holo_set_tuning_for_size": [_vp, _i, _i, _i, _i, _i, _i, _i],
     "holo_set_workspace_limit": [_vp, ctypes.c_size_t],
+    "holo_workspace_size": [_vp, ctypes.POINTER(ctypes.c_size_t)],
     "holo_get_tuning_for_size": [_vp, _i, _i, _i, ctypes.POINTER(ctypes.c_int)],
     "holo_autotune": [_vp, _vp, _i, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
     "holo_set_profiling": [_vp, _i],
@@ -157,6 +158,11 @@ class Handle:
 
     def set_workspace_limit(self, nbytes: int):
         check(self.lib.holo_set_workspace_limit(self.ptr, int(nbytes)))
+
+    def workspace_size(self) -> int:
+        out = ctypes.c_size_t()
+        check(self.lib.holo_workspace_size(self.ptr, ctypes.byref(out)))
+        return out.value
 
     def set_profiling(self, enable: bool):
         check(self.lib.holo_set_profiling(self.ptr, 1 if enable else 0))

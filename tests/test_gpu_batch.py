@@ -163,3 +163,21 @@ def test_gs_batch_non_square_vs_oracle(gpu, nx, ny, iters):
     ref = asm.phase_retrieval(i1, i2, iters, z=2000.0, wavlen=LAM, dx=DX)
     np.testing.assert_allclose(np.abs(out.cpu().numpy()), np.sqrt(i1), rtol=2e-6)
     assert asm.weighted_phase_rms(out.cpu().numpy(), ref) < 1e-3
+
+
+def test_workspace_size_is_queryable(gpu):
+    """holo_workspace_size reports the handle's device workspace, which holds at least the
+    spectrum and one round of column-transformed slices after a reconstruction."""
+    import particleholography as ph
+    from particleholography import _lib
+
+    n, slices = 256, 40
+    dsq = ph.cu_transfer_sqrt_arr(n, LAM, DX)
+    wf = ph.cu_gabor_wavefront(torch.from_numpy(_holo(n, n, 3)).to(gpu))
+    ph.cu_get_reconst_xyprojection(wf, ph.cu_transfer(-1e4, n, LAM, dsq), ph.cu_transfer(-200.0, n, LAM, dsq), slices)
+    torch.cuda.synchronize()
+    h = _lib.handle(0)
+    from particleholography.batch import get_tuning_for_size
+
+    rounds = get_tuning_for_size(n, n, slices)["batch"]
+    assert h.workspace_size() >= (1 + rounds) * n * n * 8
