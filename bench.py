@@ -59,7 +59,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the plain-path and GS figures")
     ap.add_argument("--batch-holograms", type=int, default=64, help="holograms in the batched workload")
-    ap.add_argument("--serve-requests", type=int, default=1000,
+    ap.add_argument("--serve-requests", type=int, default=5000,
                     help="requests in the served Poisson stream (the full stream has 5000)")
     ap.add_argument("--profile-gs", action="store_true",
                     help="for rocprofv3: Gerchberg-Saxton at 1024^2 (one warm-up call, then --steps "

@@ -118,6 +118,7 @@ def test_projection_configured_sizes_vs_fp64_oracle(gpu, nx, ny, slices):
     h = _holo(nx, nx + slices, ny=ny)
     want = asm.reconst_xyprojection(asm.gabor_wavefront(h), slices, "exact", z_front=z0, z_dz=dz, wavlen=LAM, dx=DX)
     got = reconst_batch([ReconJob(_t(h, gpu), z0, dz, slices, LAM, DX)])[0].cpu().numpy()
+    print(f"projection {nx}x{ny} x {slices}: rel L2 vs FP64 oracle {asm.rel_l2(got, want):.2e}")
     assert asm.rel_l2(got, want) < 1e-4
     if nx == ny:
         dsq = ph.cu_transfer_sqrt_arr(nx, LAM, DX)
@@ -180,6 +181,25 @@ def test_projection_c1_config_vs_fp64_oracle(gpu):
     err = asm.rel_l2(got, want)
     print(f"C1 512^2x100 projection rel L2 vs FP64 oracle: {err:.3e}")
     assert err < 1e-4
+
+
+def test_gs_c3_config_vs_fp64_oracle(gpu):
+    """The GS configuration: the seeded 500-disk scene recorded at planes 2000 um apart
+    (1024^2), 100 iterations on the production path (graphs, line shapes), against the FP64
+    oracle's 100 iterations: amplitude-weighted phase RMS."""
+    ph = _ph()
+    from particleholography import synth
+
+    n, z, iters = 1024, 2000.0, 100
+    i1, i2 = synth.gs_pair(103, 500, n, z)
+    dsq = ph.cu_transfer_sqrt_arr(n, LAM, DX)
+    out = ph.cu_phase_retrieval_holo(_t(i1, gpu), _t(i2, gpu), ph.cu_transfer(z, n, LAM, dsq),
+                                     ph.cu_transfer(-z, n, LAM, dsq), iters, n).data.cpu().numpy()
+    want = asm.phase_retrieval(i1, i2, iters, z=z, wavlen=LAM, dx=DX)
+    np.testing.assert_allclose(np.abs(out), np.sqrt(i1), rtol=2e-6)
+    err = asm.weighted_phase_rms(out, want)
+    print(f"C3 GS 1024^2 x 100 iterations: weighted phase RMS vs FP64 oracle {err:.2e} rad")
+    assert err < 1e-3
 
 
 # ---------------------------------------------------------------------------------------
