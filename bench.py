@@ -7,7 +7,7 @@ slices from z0 = 1e4 um in steps of 200 um; complex-FP32 FFTs, FP64-wrapped tran
 fused multiply / inverse FFT / |.|^2 / running min. One step = one full reconstruction
 (sqrt + forward spectrum + 1000 slices) of the device-resident hologram.
 
-Prints one JSON line. --gpus N > 1 is refused: this build measures one GPU.
+Prints one JSON line. --gpus N > 1 is refused: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -75,8 +75,9 @@ def main():
                          "--steps calls after one warm-up); no other work")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the z-range "
-              "sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path (a reconstruction "
+              "would shard by z-range and merge the projections with one min-reduction); run with "
+              "--gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
