@@ -34,7 +34,7 @@ enum holo_status {
   HOLO_ERR_SIZE = 2,    /* FFT length not supported */
   HOLO_ERR_TYPE = 3,    /* unsupported element type (the reference's ArgumentError) */
   HOLO_ERR_HIP = 4,     /* HIP runtime error */
-  HOLO_ERR_NOMEM = 5,   /* device allocation failed */
+  HOLO_ERR_NOMEM = 5,   /* device (or, for holo_link_rectangles, host) allocation failed */
   HOLO_ERR_BUDGET = 6   /* request exceeds the configured device-memory budget */
 };
 

@@ -11,7 +11,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <limits>
 #include <map>
+#include <new>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -1240,8 +1242,8 @@ int holo_dilate(holo_handle h, void* stream, int nx, int ny, int count, const ui
   return HOLO_OK;
 }
 
-int holo_link_rectangles(const int32_t* rects, const int64_t* offsets, int nslices, int three_d,
-                         int64_t* rows, int32_t* born, uint8_t* alive, int64_t* nrows) {
+static int link_rectangles(const int32_t* rects, const int64_t* offsets, int nslices, int three_d,
+                           int64_t* rows, int32_t* born, uint8_t* alive, int64_t* nrows) {
   if (!nrows || (nslices > 0 && !offsets)) return fail(HOLO_ERR_ARG, "NULL argument");
   if (nslices < 0) return fail(HOLO_ERR_ARG, "nslices must be >= 0");
   *nrows = 0;
@@ -1251,6 +1253,9 @@ int holo_link_rectangles(const int32_t* rects, const int64_t* offsets, int nslic
     if (offsets[s + 1] < offsets[s]) return fail(HOLO_ERR_ARG, "offsets must not decrease");
   const int64_t total = offsets[nslices];
   if (total > 0 && (!rects || !rows || !born || !alive)) return fail(HOLO_ERR_ARG, "NULL argument");
+  // rows are indexed with 32-bit integers
+  if (total > std::numeric_limits<int32_t>::max())
+    return fail(HOLO_ERR_ARG, "more than 2^31 - 1 rectangles");
   LinkTable t;
   t.reserve(static_cast<size_t>(total));
   LinkGrid grid_store;
@@ -1279,6 +1284,15 @@ int holo_link_rectangles(const int32_t* rects, const int64_t* offsets, int nslic
   }
   *nrows = static_cast<int64_t>(t.size());
   return HOLO_OK;
+}
+
+int holo_link_rectangles(const int32_t* rects, const int64_t* offsets, int nslices, int three_d,
+                         int64_t* rows, int32_t* born, uint8_t* alive, int64_t* nrows) {
+  try {
+    return link_rectangles(rects, offsets, nslices, three_d, rows, born, alive, nrows);
+  } catch (const std::bad_alloc&) {
+    return fail(HOLO_ERR_NOMEM, "host allocation failed while linking rectangles");
+  }
 }
 
 int holo_reconst_xyprojection(holo_handle h, void* stream, int nx, int ny, const void* wavefront,
