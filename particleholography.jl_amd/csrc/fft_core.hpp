@@ -259,14 +259,21 @@ struct DFT<20, DIR> {
 };
 
 // ------------------------------------------------------------------------------------------
-// LDS addressing: one padding slot every 16 elements breaks the power-of-two strides of the
-// autosort writes across banks.
+// LDS addressing: one padding slot every 2^PS elements breaks the strides of the autosort
+// writes across banks. Rows (one line per thread group, stride 1) use one slot in 32: their
+// radix-4 stages of the 768 and 1280 plans write with element stride 4, which one slot in 16
+// maps onto repeating banks (measured on the 1280 row pass: 11.1M bank-conflict cycles per
+// launch with 1/16, 6.0M with 1/32, 12% shorter). Columns (W interleaved lines) keep one in
+// 16, under which the 1024 column pass has no conflicts at all.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ int lds_pad(int n) { return n + (n >> 4); }
-__host__ __device__ constexpr int lds_padded_len(int L) { return L + (L >> 4) + 1; }
+constexpr int kRowPad = 5;
+constexpr int kColPad = 4;
+template <int PS>
+__device__ __forceinline__ int lds_pad(int n) { return n + (n >> PS); }
+__host__ __device__ constexpr int lds_padded_len(int L, int PS) { return L + (L >> PS) + 1; }
 
 // One Stockham stage: E/R butterflies per thread, outputs to LDS unless this is the last.
-template <int R, int E, int DIR>
+template <int R, int E, int DIR, int PS>
 __device__ __forceinline__ void fft_stage(float2 (&v)[E], const int t, const int T,
                                           const StagePlan sp, const float2* __restrict__ tw,
                                           float2* __restrict__ lds, const int es, const bool last) {
@@ -301,22 +308,22 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], const int t, const int
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
-      for (int m = 0; m < R; ++m) lds[lds_pad(base[q] + m * sp.p) * es] = v[q + Q * m];
+      for (int m = 0; m < R; ++m) lds[lds_pad<PS>(base[q] + m * sp.p) * es] = v[q + Q * m];
     __syncthreads();
   }
 }
 
-template <int R, int E, int DIR>
+template <int R, int E, int DIR, int PS>
 __device__ __forceinline__ void fft_stage_if(float2 (&v)[E], int t, int T, const StagePlan& sp,
                                              const float2* __restrict__ tw, float2* __restrict__ lds,
                                              int es, bool last) {
-  if constexpr (E % R == 0) fft_stage<R, E, DIR>(v, t, T, sp, tw, lds, es, last);
+  if constexpr (E % R == 0) fft_stage<R, E, DIR, PS>(v, t, T, sp, tw, lds, es, last);
 }
 
 // Full line FFT (unnormalized). DIR = -1 forward, +1 inverse. Must be called by every thread
 // of the workgroup (it contains barriers). On entry and exit v[e] holds element t + T*e.
-// lds: this line's LDS base; element n lives at lds[lds_pad(n) * es].
-template <int E, int DIR>
+// lds: this line's LDS base; element n lives at lds[lds_pad<PS>(n) * es].
+template <int E, int DIR, int PS>
 __device__ __forceinline__ void fft_line(float2 (&v)[E], float2* __restrict__ lds, const int es,
                                          const int t, const LinePlan& pl,
                                          const float2* __restrict__ tw) {
@@ -325,32 +332,32 @@ __device__ __forceinline__ void fft_line(float2 (&v)[E], float2* __restrict__ ld
     const StagePlan sp = pl.st[s];
     if (s > 0) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = lds[lds_pad(t + T * e) * es];
+      for (int e = 0; e < E; ++e) v[e] = lds[lds_pad<PS>(t + T * e) * es];
     }
     const bool last = (s + 1 == pl.ns);
     switch (sp.R) {
-      case 2: fft_stage_if<2, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 3: fft_stage_if<3, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 4: fft_stage_if<4, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 5: fft_stage_if<5, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 6: fft_stage_if<6, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 8: fft_stage_if<8, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 10: fft_stage_if<10, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 12: fft_stage_if<12, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 16: fft_stage_if<16, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
-      case 20: fft_stage_if<20, E, DIR>(v, t, T, sp, tw, lds, es, last); break;
+      case 2: fft_stage_if<2, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 3: fft_stage_if<3, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 4: fft_stage_if<4, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 5: fft_stage_if<5, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 6: fft_stage_if<6, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 8: fft_stage_if<8, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 10: fft_stage_if<10, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 12: fft_stage_if<12, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 16: fft_stage_if<16, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
+      case 20: fft_stage_if<20, E, DIR, PS>(v, t, T, sp, tw, lds, es, last); break;
       default: break;
     }
   }
 }
 
 // ------------------------------------------------------------------------------------------
-// Compile-time plans for the hot lengths: the same radix schedule and twiddle-table layout
-// as the host planner (largest radix dividing both E and the remaining length, first), with
-// every radix, span, stride and table offset a constant.
+// Radix plans: at each stage the largest radix dividing both E and the remaining length. The
+// host planner takes its schedule from the same function, so compile-time and runtime plans
+// agree on the stages and the twiddle-table layout. (Putting the radices with an odd factor
+// last was measured too: it helps the 1280 and 768 row passes with 1-in-16 padding but slows
+// the interleaved 768 column pass by 20%; with the row padding below it gains nothing.)
 // ------------------------------------------------------------------------------------------
-// Radices in the order the planner tries them (the largest dividing both E and the remaining
-// length first); the host planner uses the same list.
 constexpr int kNumRadices = 10;
 constexpr int kRadixOrder[kNumRadices] = {20, 16, 12, 10, 8, 6, 5, 4, 3, 2};
 
@@ -360,24 +367,29 @@ struct CPlanData {
   int P[kMaxStages];
   int off[kMaxStages];
   int ntw;
+  bool complete;  // the radices multiply to L (false: no plan within kMaxStages stages)
 };
 
 constexpr CPlanData make_cplan(int L, int E) {
   CPlanData d{};
-  const int* radices = kRadixOrder;
-  int rem = L, span = 1, off = 0;
-  while (rem > 1 && d.ns < kMaxStages) {
+  int radix[kMaxStages] = {};
+  int n = 0, rem = L;
+  while (rem > 1 && n < kMaxStages) {
     int R = 0;
     for (int i = 0; i < kNumRadices && R == 0; ++i)
-      if (E % radices[i] == 0 && rem % radices[i] == 0) R = radices[i];
+      if (E % kRadixOrder[i] == 0 && rem % kRadixOrder[i] == 0) R = kRadixOrder[i];
     if (R == 0) break;
-    d.R[d.ns] = R;
-    d.P[d.ns] = span;
-    d.off[d.ns] = off;
-    if (span > 1) off += (R - 1) * span;
-    ++d.ns;
-    span *= R;
+    radix[n++] = R;
     rem /= R;
+  }
+  d.complete = rem == 1;
+  for (int i = 0; i < n; ++i) d.R[d.ns++] = radix[i];
+  int span = 1, off = 0;
+  for (int s = 0; s < d.ns; ++s) {
+    d.P[s] = span;
+    d.off[s] = off;
+    if (span > 1) off += (d.R[s] - 1) * span;
+    span *= d.R[s];
   }
   d.ntw = off;
   return d;
@@ -389,7 +401,7 @@ struct CPlan {
   static constexpr int T = L / E;
 };
 
-template <int R, int E, int DIR, int T, int P, int OFF, bool LAST>
+template <int R, int E, int DIR, int T, int P, int OFF, bool LAST, int PS>
 __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const float2* __restrict__ tw,
                                              float2* __restrict__ lds, const int es) {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
@@ -423,12 +435,12 @@ __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const 
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
-      for (int m = 0; m < R; ++m) lds[lds_pad(base[q] + m * P) * es] = v[q + Q * m];
+      for (int m = 0; m < R; ++m) lds[lds_pad<PS>(base[q] + m * P) * es] = v[q + Q * m];
     __syncthreads();
   }
 }
 
-template <int E, int DIR, int L, int S = 0>
+template <int E, int DIR, int L, int PS, int S = 0>
 __device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__ lds, const int es,
                                             const int t, const float2* __restrict__ tw) {
   constexpr CPlanData d = CPlan<L, E>::d;
@@ -436,22 +448,33 @@ __device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__
     constexpr int T = CPlan<L, E>::T;
     if constexpr (S > 0) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = lds[lds_pad(t + T * e) * es];
+      for (int e = 0; e < E; ++e) v[e] = lds[lds_pad<PS>(t + T * e) * es];
     }
-    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns>(v, t, tw, lds, es);
-    fft_line_ct<E, DIR, L, S + 1>(v, lds, es, t, tw);
+    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns, PS>(v, t, tw, lds, es);
+    fft_line_ct<E, DIR, L, PS, S + 1>(v, lds, es, t, tw);
   }
 }
 
 // Line FFT with a compile-time plan when L != 0, else the runtime plan `pl`.
-template <int E, int DIR, int L>
+template <int E, int DIR, int L, int PS>
 __device__ __forceinline__ void fft_run(float2 (&v)[E], float2* __restrict__ lds, const int es,
                                         const int t, const LinePlan& pl, const float2* __restrict__ tw) {
   if constexpr (L != 0) {
-    fft_line_ct<E, DIR, L>(v, lds, es, t, tw);
+    fft_line_ct<E, DIR, L, PS>(v, lds, es, t, tw);
   } else {
-    fft_line<E, DIR>(v, lds, es, t, pl, tw);
+    fft_line<E, DIR, PS>(v, lds, es, t, pl, tw);
   }
+}
+// A row line (stride 1 in LDS) and a column line of a tile of W interleaved columns.
+template <int E, int DIR, int L>
+__device__ __forceinline__ void fft_row(float2 (&v)[E], float2* __restrict__ line, const int t,
+                                        const LinePlan& pl, const float2* __restrict__ tw) {
+  fft_run<E, DIR, L, kRowPad>(v, line, 1, t, pl, tw);
+}
+template <int E, int DIR, int L>
+__device__ __forceinline__ void fft_col(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,
+                                        const LinePlan& pl, const float2* __restrict__ tw) {
+  fft_run<E, DIR, L, kColPad>(v, col, W, t, pl, tw);
 }
 
 // Threads per line: a constant when the plan is compile-time.

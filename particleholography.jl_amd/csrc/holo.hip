@@ -206,24 +206,17 @@ int get_plan(holo_context* h, int L, const Plan1D** out, int E_req = 0) {
   p.lp.L = L;
   p.lp.T = L / E;
   p.lp.ns = 0;
-  const int* kRadices = kRadixOrder;
-  int rem = L, span = 1, twoff = 0;
+  // the schedule of the device's compile-time plans (fft_core.hpp: make_cplan)
+  const CPlanData c = make_cplan(L, E);
+  if (!c.complete) return fail(HOLO_ERR_SIZE, "no radix plan for length " + std::to_string(L));
   std::vector<float2> tw;
-  while (rem > 1) {
-    int R = 0;
-    for (int ri = 0; ri < kNumRadices; ++ri) {
-      const int r = kRadices[ri];
-      if (E % r == 0 && rem % r == 0) {
-        R = r;
-        break;
-      }
-    }
-    if (R == 0 || p.lp.ns >= kMaxStages) return fail(HOLO_ERR_SIZE, "no radix plan for length " + std::to_string(L));
+  for (int s = 0; s < c.ns; ++s) {
     StagePlan& sp = p.lp.st[p.lp.ns++];
+    const int R = c.R[s], span = c.P[s];
     sp.R = R;
     sp.p = span;
     sp.pmagic = span >= 2 ? static_cast<uint32_t>((0x100000000ull + span - 1) / span) : 0u;
-    sp.twoff = twoff;
+    sp.twoff = c.off[s];
     if (span > 1) {
       const long long PR = static_cast<long long>(span) * R;
       for (int m = 1; m < R; ++m)
@@ -232,19 +225,10 @@ int get_plan(holo_context* h, int L, const Plan1D** out, int E_req = 0) {
           const double a = -2.0 * M_PI * static_cast<double>(mk) / static_cast<double>(PR);
           tw.push_back(make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a))));
         }
-      twoff += (R - 1) * span;
     }
-    span *= R;
-    rem /= R;
   }
-  p.lp.ntw = static_cast<int>(tw.size());
-  {  // the device's compile-time plans must describe the same schedule and table layout
-    const CPlanData c = make_cplan(L, E);
-    bool same = c.ns == p.lp.ns && c.ntw == p.lp.ntw;
-    for (int i = 0; same && i < c.ns; ++i)
-      same = c.R[i] == p.lp.st[i].R && c.P[i] == p.lp.st[i].p && c.off[i] == p.lp.st[i].twoff;
-    if (!same) return fail(HOLO_ERR_SIZE, "internal: plan mismatch for length " + std::to_string(L));
-  }
+  p.lp.ntw = c.ntw;
+  if (static_cast<int>(tw.size()) != c.ntw) return fail(HOLO_ERR_SIZE, "internal: twiddle table size");
   if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
   HOLO_HIP(hipMalloc(&p.dtw, tw.size() * sizeof(float2)));
   HOLO_HIP(hipMemcpy(p.dtw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
@@ -266,10 +250,10 @@ int ilog2(int v) {
 
 // dynamic LDS: the lines of the tile, then the plan's twiddle table
 size_t row_lds(const Geom& g) {
-  return (static_cast<size_t>(g.rows) * lds_padded_len(g.nx) + g.px.ntw) * sizeof(float2);
+  return (static_cast<size_t>(g.rows) * lds_padded_len(g.nx, kRowPad) + g.px.ntw) * sizeof(float2);
 }
 size_t col_lds(const Geom& g) {
-  return (static_cast<size_t>(g.W) * lds_padded_len(g.ny) + g.py.ntw) * sizeof(float2);
+  return (static_cast<size_t>(g.W) * lds_padded_len(g.ny, kColPad) + g.py.ntw) * sizeof(float2);
 }
 
 // Effective tuning for a size: the per-size entry, with every nonzero field of the global
@@ -337,7 +321,7 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
   g->twy = py->dtw;
   *Ex = px->E;
   *Ey = py->E;
-  const size_t colbytes = static_cast<size_t>(lds_padded_len(ny)) * sizeof(float2);
+  const size_t colbytes = static_cast<size_t>(lds_padded_len(ny, kColPad)) * sizeof(float2);
   const size_t coltw = static_cast<size_t>(py->lp.ntw) * sizeof(float2);
   const size_t maxl = static_cast<size_t>(h->max_lds);
   int W = tn.cols;
@@ -351,7 +335,7 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
     return fail(HOLO_ERR_ARG, "column tile too large for a workgroup");
   g->W = W;
   g->wshift = ilog2(W);
-  const size_t rowbytes = static_cast<size_t>(lds_padded_len(nx)) * sizeof(float2);
+  const size_t rowbytes = static_cast<size_t>(lds_padded_len(nx, kRowPad)) * sizeof(float2);
   const size_t rowtw = static_cast<size_t>(px->lp.ntw) * sizeof(float2);
   int rows = tn.rows;
   if (rows <= 0) rows = std::max(1, row_threads / px->lp.T);
@@ -1072,17 +1056,21 @@ int ccl_source(int kind, const void* img, double threshold, int nx, CclSrc* src)
   return HOLO_OK;
 }
 
+}  // extern "C"
+
 struct CclGrid {
   dim3 tiles;       // (ceil(nx/64), ceil(ny/16), slices of the round)
   size_t per_slice; // tiles per slice
 };
 
-CclGrid ccl_grid(int nx, int ny, int c) {
+static CclGrid ccl_grid(int nx, int ny, int c) {
   CclGrid g;
   g.tiles = dim3((nx + kTileX - 1) / kTileX, (ny + kTileY - 1) / kTileY, c);
   g.per_slice = static_cast<size_t>(g.tiles.x) * g.tiles.y;
   return g;
 }
+
+extern "C" {
 
 // Slices per round so that the union-find state (P and the root slots M, 8 B/px, plus the
 // tile flags and list) stays within 1 GiB; at least one. P and M are written only in tiles

@@ -177,8 +177,8 @@ __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
   c.w = threadIdx.x / T;
   c.j = blockIdx.x * g.rows + c.w;
   c.ok = c.j < g.ny;
-  c.line = lds + c.w * lds_padded_len(g.nx);
-  float2* tw = lds + g.rows * lds_padded_len(g.nx);
+  c.line = lds + c.w * lds_padded_len(g.nx, kRowPad);
+  float2* tw = lds + g.rows * lds_padded_len(g.nx, kRowPad);
   if (STAGE) stage_twiddles(tw, g.twx, g.px.ntw);
   c.tw = tw;
   return c;
@@ -207,7 +207,7 @@ __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   c.t = threadIdx.x >> g.wshift;
   c.i = xcd_tile(blockIdx.x, gridDim.x) * g.W + c.w;
   c.col = lds + c.w;
-  float2* tw = lds + g.W * lds_padded_len(g.ny);
+  float2* tw = lds + g.W * lds_padded_len(g.ny, kColPad);
   if (STAGE) stage_twiddles(tw, g.twy, g.py.ntw);
   c.tw = tw;
   return c;
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(512) void k_rows_fft(const void* __restrict__ in, s
       v[e] = make_float2(sqrtf(static_cast<const float*>(in)[zin + rowoff + i]), 0.f);
     }
   }
-  fft_run<E, DIR, L>(v, c.line, 1, c.t, g.px, c.tw);
+  fft_row<E, DIR, L>(v, c.line, c.t, g.px, c.tw);
   if (c.ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e) out[zout + rowoff + c.t + T * e] = cscale(v[e], scale);
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(512) void k_cols_fft(const float2* __restrict__ in,
 #pragma unroll
   for (int e = 0; e < E; ++e)
     v[e] = ok ? in[zin + static_cast<size_t>(c.t + T * e) * g.nx + c.i] : make_float2(0.f, 0.f);
-  fft_run<E, DIR, L>(v, c.col, g.W, c.t, g.py, c.tw);
+  fft_col<E, DIR, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if (ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
 #pragma unroll
         for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
       }
-      fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+      fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
       if (ok) {
         float2* __restrict__ dst = G + static_cast<size_t>(k) * a.gstride;
 #pragma unroll
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
       float2 v[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = f[e];
-      fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+      fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
       if (ok) {
         float2* __restrict__ dst = a.G + static_cast<size_t>(k) * a.gstride;
 #pragma unroll
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
     const float2* __restrict__ src = G + static_cast<size_t>(k) * gstride + rowoff;
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = c.ok ? src[c.t + T * e] : make_float2(0.f, 0.f);
-    fft_run<E, +1, L>(v, c.line, 1, c.t, g.px, c.tw);
+    fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float a = v[e].x * sc, b = v[e].y * sc;
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
     const float2* __restrict__ src = a.G + static_cast<size_t>(k) * a.gstride + rowoff;
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = c.ok ? src[c.t + T * e] : make_float2(0.f, 0.f);
-    fft_run<E, +1, L>(v, c.line, 1, c.t, g.px, c.tw);
+    fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
     if (rowin) {
       const size_t sbase = static_cast<size_t>(a.slice0 + k) * a.vstride + static_cast<size_t>(oj) * a.onx;
 #pragma unroll
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
     if constexpr (HMODE == 3) lv[e] = a.lpf[centred_index(ci, c.t + T * e, g.nx, g.ny)];
   }
   stage_twiddles(c.tw, g.twy, g.py.ntw);
-  fft_run<E, -1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+  fft_col<E, -1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if constexpr (HMODE == 0) {
     const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
 #pragma unroll
@@ -647,7 +647,7 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = make_float2(lv[e] * v[e].x, lv[e] * v[e].y);
   }
-  fft_run<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+  fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if (ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e) dst[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(512) void k_gs_rows(GsTable gt, float sc, Geom g) {
     iv[e] = c.ok ? I[rowoff + c.t + T * e] : 0.f;
   }
   stage_twiddles(c.tw, g.twx, g.px.ntw);
-  fft_run<E, +1, L>(v, c.line, 1, c.t, g.px, c.tw);
+  fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const float A = sqrtf(iv[e]);
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(512) void k_gs_rows(GsTable gt, float sc, Geom g) {
       v[e] = make_float2(A, 0.f);
     }
   }
-  if (!final) fft_run<E, -1, L>(v, c.line, 1, c.t, g.px, c.tw);
+  if (!final) fft_row<E, -1, L>(v, c.line, c.t, g.px, c.tw);
   if (c.ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e) out[rowoff + c.t + T * e] = v[e];
