@@ -273,7 +273,9 @@ int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_c
 /* The effective choices for a size (defaults resolved) as {batch, spg_cols, spg_rows, cols, rows}. */
 int holo_get_tuning_for_size(holo_handle h, int nx, int ny, int slices, int* out5);
 /* Times candidate tile configurations of both reconstruction passes on the device (HIP events,
- * synchronous) and keeps the fastest for this size; out5 receives it (may be NULL). */
+ * synchronous) and keeps the fastest for this size; the tiles in force before the call stay
+ * unless the winner is at least 2% faster. out5 (may be NULL) receives the tiles in force
+ * afterwards, defaults resolved as by holo_get_tuning_for_size with slices = 0. */
 int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, double dx, int* out5);
 
 /* ---- measurement (no counterpart in the reference, which has no timers) ---------------- */

@@ -1641,7 +1641,9 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     int Ex, Ey;
     int r = make_geom(h, nx, ny, &g, &Ex, &Ey, 512);
     if (r != HOLO_OK) return r;
-    const ReconPlan rp = recon_plan(h, g, t.batch);
+    // batch 0 (the default round, e.g. an incumbent without a size entry) is timed at the
+    // round size a long reconstruction would use, not capped at zero slices
+    const ReconPlan rp = recon_plan(h, g, t.batch > 0 ? t.batch : 1 << 20);
     HOLO_TRY(ensure_ws(h, pbytes * (1 + static_cast<size_t>(rp.m)) + plane * sizeof(float)));
     float2* F = static_cast<float2*>(h->ws);
     float* proj = reinterpret_cast<float*>(F + plane * (1 + rp.m));
@@ -1727,14 +1729,8 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     return fail(HOLO_ERR_SIZE, "no tile configuration fits this size");
   }
   h->size_tune[{nx, ny}] = best;
-  if (out5) {
-    out5[0] = best.batch;
-    out5[1] = best.spg_cols;
-    out5[2] = best.spg_rows;
-    out5[3] = best.cols;
-    out5[4] = best.rows;
-  }
-  return HOLO_OK;
+  // the tiles in force, with defaults resolved (an incumbent without a size entry keeps none)
+  return out5 ? holo_get_tuning_for_size(h, nx, ny, 0, out5) : HOLO_OK;
 }
 
 int holo_workspace_size(holo_handle h, size_t* bytes) {
