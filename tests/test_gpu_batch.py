@@ -87,6 +87,9 @@ def test_autotune_keeps_results_correct(gpu):
     assert choice["batch"] >= 1 and choice["cols_per_wg"] in (1, 2, 4, 8, 16)
     eff = batch.get_tuning_for_size(n, n, 1000)
     assert eff["batch"] == choice["batch"] and eff["spg_cols"] == choice["spg_cols"]
+    # the reported choice is the tiles in force, defaults resolved (also when the incumbent,
+    # the default round of a size without an entry, keeps its place)
+    assert batch.get_tuning_for_size(n, n, 0) == choice
     try:
         h = _holo(n, n, 9)
         ht = torch.from_numpy(h).to(gpu)
