@@ -1016,7 +1016,10 @@ int holo_make_background(holo_handle h, void* stream, int nx, int ny, int count,
     if (count > 65535) return fail(HOLO_ERR_SIZE, "at most 65535 images per background-mode call");
     const int vec = (npx % 2 == 0) && (reinterpret_cast<uintptr_t>(stack) % 2 == 0);
     const unsigned gx = static_cast<unsigned>((npx + 2 * kModePairs - 1) / (2 * kModePairs));
-    hipLaunchKernelGGL(k_background_mode, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out, vec);
+    if (vec)
+      hipLaunchKernelGGL(k_background_mode<true>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
+    else
+      hipLaunchKernelGGL(k_background_mode<false>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
   } else if (kind == HOLO_BACKGROUND_MEAN) {
     const int vec = (npx % 4 == 0) && (reinterpret_cast<uintptr_t>(stack) % 4 == 0);
     const size_t threads = vec ? npx / 4 : npx;
