@@ -159,12 +159,19 @@ __global__ void k_lpf_super_gaussian(int n, double sigma, double dx, float* __re
 // ------------------------------------------------------------------------------------------
 // helpers shared by the pass kernels
 // ------------------------------------------------------------------------------------------
+// Loads of out-of-range lines (the last row or column tile of a plane) read a clamped, valid
+// line instead of being skipped: a load under a per-thread branch is followed by a full wait
+// for memory, which serialises a thread's loads. Those lines' results are never stored.
+// Single-pass kernels issue their data loads before copying the twiddle table into LDS, so that
+// one memory latency covers both; the slice-looping row passes (projection, store) stage the
+// table first (staging inside their slice loop cost the projection pass 35%).
 // LDS of a row pass: `rows` padded lines, then the x plan's twiddle table.
 // STAGE false leaves the twiddle copy to the kernel, which then issues its global loads
 // first: the copy's LDS writes wait for the table, and a single-plane launch (two waves per
 // SIMD) cannot hide that latency behind other waves.
 struct RowCtx {
   int t, w, j;
+  int jl;  // j clamped into the plane: loads read row jl unconditionally (see below)
   bool ok;
   float2* line;
   float2* tw;
@@ -177,6 +184,7 @@ __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
   c.w = threadIdx.x / T;
   c.j = blockIdx.x * g.rows + c.w;
   c.ok = c.j < g.ny;
+  c.jl = c.ok ? c.j : g.ny - 1;
   c.line = lds + c.w * lds_padded_len(g.nx, kRowPad);
   float2* tw = lds + g.rows * lds_padded_len(g.nx, kRowPad);
   if (STAGE) stage_twiddles(tw, g.twx, g.px.ntw);
@@ -224,22 +232,21 @@ __global__ __launch_bounds__(512) void k_rows_fft(const void* __restrict__ in, s
                                                   float2* __restrict__ out, size_t out_stride,
                                                   float scale, Geom g) {
   extern __shared__ float2 lds[];
-  const RowCtx c = row_ctx<E, L>(g, lds);
+  const RowCtx c = row_ctx<E, L, false>(g, lds);
   const int T = line_threads<E, L>(g.px);
-  const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
+  const size_t rowoff = static_cast<size_t>(c.jl) * g.nx;
   const size_t zin = blockIdx.y * in_stride, zout = blockIdx.y * out_stride;
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = c.t + T * e;
-    if (!c.ok) {
-      v[e] = make_float2(0.f, 0.f);
-    } else if (SRC == 0) {
+    if (SRC == 0) {
       v[e] = static_cast<const float2*>(in)[zin + rowoff + i];
     } else {
       v[e] = make_float2(sqrtf(static_cast<const float*>(in)[zin + rowoff + i]), 0.f);
     }
   }
+  stage_twiddles(c.tw, g.twx, g.px.ntw);
   fft_row<E, DIR, L>(v, c.line, c.t, g.px, c.tw);
   if (c.ok) {
 #pragma unroll
@@ -253,14 +260,15 @@ __global__ __launch_bounds__(512) void k_cols_fft(const float2* __restrict__ in,
                                                   float2* __restrict__ out, size_t out_stride,
                                                   float scale, Geom g) {
   extern __shared__ float2 lds[];
-  const ColCtx c = col_ctx<E, L>(g, lds);
+  const ColCtx c = col_ctx<E, L, false>(g, lds);
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
+  const int ci = ok ? c.i : 0;
   const size_t zin = blockIdx.y * in_stride, zout = blockIdx.y * out_stride;
   float2 v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    v[e] = ok ? in[zin + static_cast<size_t>(c.t + T * e) * g.nx + c.i] : make_float2(0.f, 0.f);
+  for (int e = 0; e < E; ++e) v[e] = in[zin + static_cast<size_t>(c.t + T * e) * g.nx + ci];
+  stage_twiddles(c.tw, g.twy, g.py.ntw);
   fft_col<E, DIR, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if (ok) {
 #pragma unroll
@@ -323,13 +331,14 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
   const JobDesc& jd = jt.d[HMODE == 0 ? blockIdx.z : 0];
   if (HMODE == 0 && static_cast<int>(blockIdx.y) * a.spg >= jd.nbatch) return;
   const float2* __restrict__ Fsrc = HMODE == 0 ? jd.F : a.F;
-  const ColCtx c = col_ctx<E, L>(g, lds);
+  const ColCtx c = col_ctx<E, L, false>(g, lds);
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
   float2 f[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) f[e] = ok ? Fsrc[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
+  for (int e = 0; e < E; ++e) f[e] = Fsrc[static_cast<size_t>(c.t + T * e) * g.nx + ci];
+  if constexpr (HMODE == 0) stage_twiddles(c.tw, g.twy, g.py.ntw);
 
   if constexpr (HMODE == 0) {
     // f <- F .* H_{k0} and r = exp(2 pi i dzl sqrt(d)), both from FP64-wrapped phases; then
@@ -374,14 +383,19 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
   } else {
     float2 hdz[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int j = c.t + T * e;
-      hdz[e] = a.Hdz[centred_index(ci, j, g.nx, g.ny)];
-      if (a.first) {
-        f[e] = cmul(f[e], a.H0[centred_index(ci, j, g.nx, g.ny)]);
-      } else {
-        f[e] = ok ? a.run[static_cast<size_t>(j) * g.nx + ci] : make_float2(0.f, 0.f);
-      }
+    for (int e = 0; e < E; ++e) hdz[e] = a.Hdz[centred_index(ci, c.t + T * e, g.nx, g.ny)];
+    // one branch around all loads of each kind (a branch per load costs a wait per load)
+    if (a.first) {
+      float2 h0[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) h0[e] = a.H0[centred_index(ci, c.t + T * e, g.nx, g.ny)];
+      stage_twiddles(c.tw, g.twy, g.py.ntw);
+#pragma unroll
+      for (int e = 0; e < E; ++e) f[e] = cmul(f[e], h0[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) f[e] = a.run[static_cast<size_t>(c.t + T * e) * g.nx + ci];
+      stage_twiddles(c.tw, g.twy, g.py.ntw);
     }
     for (int k = 0; k < a.nbatch; ++k) {
       if (k > 0 || !a.first) {
@@ -614,8 +628,7 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
   const int ci = ok ? c.i : 0;
   float2 v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    v[e] = ok ? src[static_cast<size_t>(c.t + T * e) * g.nx + ci] : make_float2(0.f, 0.f);
+  for (int e = 0; e < E; ++e) v[e] = src[static_cast<size_t>(c.t + T * e) * g.nx + ci];
   // the transfer (or filter) values are loaded with the column, in the same memory latency
   constexpr bool kHArr = HMODE == 1 || HMODE == 2;
   float2 hv[kHArr ? E : 1];
@@ -668,13 +681,13 @@ __global__ __launch_bounds__(512) void k_gs_rows(GsTable gt, float sc, Geom g) {
   float2* __restrict__ out = final ? rd.out : rd.B;
   const RowCtx c = row_ctx<E, L, false>(g, lds);
   const int T = line_threads<E, L>(g.px);
-  const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
+  const size_t rowoff = static_cast<size_t>(c.jl) * g.nx;
   float2 v[E];
   float iv[E];  // the intensity is loaded with the row, in the same memory latency
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    v[e] = c.ok ? U[rowoff + c.t + T * e] : make_float2(0.f, 0.f);
-    iv[e] = c.ok ? I[rowoff + c.t + T * e] : 0.f;
+    v[e] = U[rowoff + c.t + T * e];
+    iv[e] = I[rowoff + c.t + T * e];
   }
   stage_twiddles(c.tw, g.twx, g.px.ntw);
   fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
