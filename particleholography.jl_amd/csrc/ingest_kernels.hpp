@@ -72,7 +72,7 @@ __global__ void k_ingest_u8(size_t npx, int count, const uint8_t* __restrict__ s
 // VEC: npx even and the stack 2-byte aligned, one uchar2 load per pair and image.
 constexpr int kModePairs = 64;
 constexpr int kModeThreads = 256;
-constexpr int kModeUnroll = 16;
+constexpr int kModeUnroll = 32;
 template <bool VEC>
 __global__ __launch_bounds__(kModeThreads) void k_background_mode(size_t npx, int count,
                                                                    const uint8_t* __restrict__ stack,
