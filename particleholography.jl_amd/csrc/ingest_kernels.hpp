@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kModeThreads) void k_background_mode(size_t npx, in
 // divided by count (make_background :mean, utils.jl:219-224). With `vec` (npx % 4 == 0 and
 // the stack 4-byte aligned) a thread owns 4 consecutive pixels and reads one uchar4 per
 // image, else one pixel; kMeanUnroll images are loaded before they are added in order.
-constexpr int kMeanUnroll = 8;
+constexpr int kMeanUnroll = 16;
 __global__ void k_background_mean(size_t npx, int count, const uint8_t* __restrict__ stack,
                                   double* __restrict__ out, int vec) {
 #pragma clang fp contract(off)
