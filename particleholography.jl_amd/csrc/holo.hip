@@ -999,7 +999,11 @@ int holo_ingest_u8(holo_handle h, void* stream, int nx, int ny, int count, const
     float* o = out + static_cast<size_t>(m0) * npx;
     const int vec = (npx % 4 == 0) && (reinterpret_cast<uintptr_t>(s) % 4 == 0) &&
                     (reinterpret_cast<uintptr_t>(o) % 16 == 0);
-    hipLaunchKernelGGL(k_ingest_u8, dim3(gx, c), dim3(256), 0, st, npx, c, s, bkg, offset, o, vec);
+    const dim3 grid(gx, c), block(256);
+    if (vec && bkg) hipLaunchKernelGGL((k_ingest_u8<true, true>), grid, block, 0, st, npx, c, s, bkg, offset, o);
+    else if (vec) hipLaunchKernelGGL((k_ingest_u8<true, false>), grid, block, 0, st, npx, c, s, bkg, offset, o);
+    else if (bkg) hipLaunchKernelGGL((k_ingest_u8<false, true>), grid, block, 0, st, npx, c, s, bkg, offset, o);
+    else hipLaunchKernelGGL((k_ingest_u8<false, false>), grid, block, 0, st, npx, c, s, bkg, offset, o);
     HOLO_TRY(check_launch());
   }
   return HOLO_OK;

@@ -21,11 +21,12 @@ __device__ __forceinline__ float n0f8_to_f32(unsigned int v) {
 
 // out = Float32((Float64(v / 255) - bkg) + offset), v / 255 rounded to Float32 first;
 // bkg NULL means 0, so with offset 0 this is exactly v / 255. Four pixels per thread, loaded
-// as one uchar4 and stored as one float4 when `vec` (npx % 4 == 0 and the host found src
-// 4-byte and out 16-byte aligned).
+// as one uchar4 and stored as one float4 when VEC (npx % 4 == 0 and the host found src
+// 4-byte and out 16-byte aligned). All loads read valid (clamped) addresses without a branch
+// each, so the image bytes and the background values are in flight together.
+template <bool VEC, bool BKG>
 __global__ void k_ingest_u8(size_t npx, int count, const uint8_t* __restrict__ src,
-                            const double* __restrict__ bkg, double offset, float* __restrict__ out,
-                            int vec) {
+                            const double* __restrict__ bkg, double offset, float* __restrict__ out) {
 #pragma clang fp contract(off)
   const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const size_t m = blockIdx.y;
@@ -36,22 +37,25 @@ __global__ void k_ingest_u8(size_t npx, int count, const uint8_t* __restrict__ s
   if (i0 >= npx) return;
   const int n = npx - i0 >= 4 ? 4 : static_cast<int>(npx - i0);
   uint8_t v[4];
-  if (vec) {
+  double b[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (VEC) {
     const uchar4 w = *reinterpret_cast<const uchar4*>(s + i0);
     v[0] = w.x; v[1] = w.y; v[2] = w.z; v[3] = w.w;
   } else {
-    for (int k = 0; k < n; ++k) v[k] = s[i0 + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = s[min(i0 + k, npx - 1)];
+  }
+  if constexpr (BKG) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = bkg[min(i0 + k, npx - 1)];
   }
   float r[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    if (k < n) {
-      const float f = n0f8_to_f32(v[k]);
-      const double b = bkg ? bkg[i0 + k] : 0.0;
-      r[k] = static_cast<float>((static_cast<double>(f) - b) + offset);
-    }
+    const float f = n0f8_to_f32(v[k]);
+    r[k] = static_cast<float>((static_cast<double>(f) - b[k]) + offset);
   }
-  if (vec) {
+  if constexpr (VEC) {
     *reinterpret_cast<float4*>(o + i0) = make_float4(r[0], r[1], r[2], r[3]);
   } else {
     for (int k = 0; k < n; ++k) o[i0 + k] = r[k];
