@@ -1018,8 +1018,17 @@ int holo_make_background(holo_handle h, void* stream, int nx, int ny, int count,
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (kind == HOLO_BACKGROUND_MODE) {
     if (count > 65535) return fail(HOLO_ERR_SIZE, "at most 65535 images per background-mode call");
-    const int vec = (npx % 2 == 0) && (reinterpret_cast<uintptr_t>(stack) % 2 == 0);
     const unsigned gx = static_cast<unsigned>((npx + 2 * kModePairs - 1) / (2 * kModePairs));
+    static_assert(2 * kModePairs == 4 * kMode8Quads, "both mode kernels take 128-pixel tiles");
+    if (count <= 256) {  // 8-bit counters
+      const bool vec4 = (npx % 4 == 0) && (reinterpret_cast<uintptr_t>(stack) % 4 == 0);
+      if (vec4)
+        hipLaunchKernelGGL(k_background_mode8<true>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
+      else
+        hipLaunchKernelGGL(k_background_mode8<false>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
+      return check_launch();
+    }
+    const int vec = (npx % 2 == 0) && (reinterpret_cast<uintptr_t>(stack) % 2 == 0);
     if (vec)
       hipLaunchKernelGGL(k_background_mode<true>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
     else

@@ -137,6 +137,89 @@ __global__ __launch_bounds__(kModeThreads) void k_background_mode(size_t npx, in
   }
 }
 
+// The same mode for stacks of at most 256 images with 8-bit counters: a word holds the counters
+// of four consecutive pixels, so a 128-pixel tile needs 32 KB of LDS and four workgroups fit a
+// CU (twice the loads in flight of the 16-bit kernel). Thread (quad p = t % 32, phase
+// q = t / 32) counts images q, q+8, ... below 255, so no counter passes 255; the 256th image,
+// if any, is added in registers while phase q scans bins [32 q, 32 q + 32). VEC4: npx % 4 == 0
+// and the stack 4-byte aligned, one uchar4 load per quad and image.
+constexpr int kMode8Quads = 32;
+constexpr int kMode8Phases = kModeThreads / kMode8Quads;  // 8
+constexpr int kMode8Unroll = 16;
+template <bool VEC4>
+__global__ __launch_bounds__(kModeThreads) void k_background_mode8(size_t npx, int count,
+                                                                    const uint8_t* __restrict__ stack,
+                                                                    double* __restrict__ out) {
+  __shared__ unsigned int hist[256 * kMode8Quads];
+  __shared__ unsigned int best[kMode8Phases][4 * kMode8Quads];
+  const int t = threadIdx.x;
+  for (int k = t; k < 256 * kMode8Quads; k += kModeThreads) hist[k] = 0u;
+  __syncthreads();
+  const int p = t % kMode8Quads;
+  const int q = t / kMode8Quads;
+  const size_t i0 = static_cast<size_t>(blockIdx.x) * (4 * kMode8Quads) + 4 * p;
+  bool has[4];
+  size_t ix[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    has[c] = i0 + c < npx;
+    ix[c] = has[c] ? i0 + c : 0;  // loads read valid addresses without a branch each
+  }
+  const int counted = min(count, 255);
+  for (int m0 = q; m0 < counted; m0 += kMode8Phases * kMode8Unroll) {
+    uchar4 w[kMode8Unroll];
+#pragma unroll
+    for (int u = 0; u < kMode8Unroll; ++u) {
+      const uint8_t* s = stack + static_cast<size_t>(min(m0 + kMode8Phases * u, counted - 1)) * npx;
+      if constexpr (VEC4) {
+        w[u] = *reinterpret_cast<const uchar4*>(s + ix[0]);  // i0 % 4 == 0: has[0] implies has[3]
+      } else {
+        w[u] = make_uchar4(s[ix[0]], s[ix[1]], s[ix[2]], s[ix[3]]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kMode8Unroll; ++u) {
+      if (m0 + kMode8Phases * u < counted) {
+        if (has[0]) atomicAdd(&hist[w[u].x * kMode8Quads + p], 1u);
+        if (has[1]) atomicAdd(&hist[w[u].y * kMode8Quads + p], 1u << 8);
+        if (has[2]) atomicAdd(&hist[w[u].z * kMode8Quads + p], 1u << 16);
+        if (has[3]) atomicAdd(&hist[w[u].w * kMode8Quads + p], 1u << 24);
+      }
+    }
+  }
+  // the 256th image (count == 256), added during the scan
+  unsigned last[4] = {256u, 256u, 256u, 256u};  // 256: no bin
+  if (count > 255) {
+    const uint8_t* s = stack + static_cast<size_t>(255) * npx;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) last[c] = s[ix[c]];
+  }
+  __syncthreads();
+  unsigned cb[4] = {0u, 0u, 0u, 0u}, bb[4] = {0u, 0u, 0u, 0u};
+  for (int b = 32 * q; b < 32 * q + 32; ++b) {
+    const unsigned int w = hist[b * kMode8Quads + p];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const unsigned n = ((w >> (8 * c)) & 0xffu) + (last[c] == static_cast<unsigned>(b) ? 1u : 0u);
+      if (n > cb[c]) { cb[c] = n; bb[c] = b; }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) best[q][4 * p + c] = (cb[c] << 8) | bb[c];
+  __syncthreads();
+  if (t < 4 * kMode8Quads) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * (4 * kMode8Quads) + t;
+    if (i < npx) {
+      unsigned int c = 0, bv = 0;
+      for (int r = 0; r < kMode8Phases; ++r) {  // earlier phases hold smaller bins: strict > keeps them
+        const unsigned int e = best[r][t];
+        if ((e >> 8) > c) { c = e >> 8; bv = e & 0xffu; }
+      }
+      out[i] = static_cast<double>(bv) / 255.0;
+    }
+  }
+}
+
 // Per-pixel mean of a stack: sum of Float64(Float32(v / 255)) in image order from 0.0, then
 // divided by count (make_background :mean, utils.jl:219-224). With `vec` (npx % 4 == 0 and
 // the stack 4-byte aligned) a thread owns 4 consecutive pixels and reads one uchar4 per

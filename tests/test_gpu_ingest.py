@@ -35,7 +35,11 @@ def test_background_mode_known_answer(gpu):
 
 @pytest.mark.parametrize("count,ny,nx,alphabet", [
     (1, 4, 4, None), (2, 3, 5, None), (7, 33, 17, [0, 1, 2]), (64, 1024, 1024, [3, 7, 9, 200]),
-    (9, 768, 768, None), (300, 64, 96, [0, 128, 255]), (5, 1, 1, [1, 2])])
+    (9, 768, 768, None), (300, 64, 96, [0, 128, 255]), (5, 1, 1, [1, 2]),
+    # 8-bit counters up to 256 images: a counter at its 255 limit, 256 equal values (the
+    # 256th image is counted in registers), ties at 256, odd sizes (byte loads); 257: 16-bit
+    (255, 8, 8, [9]), (256, 8, 12, [9]), (256, 33, 20, [1, 2]), (256, 64, 64, None),
+    (256, 3, 5, [0, 1]), (200, 7, 9, [250, 251, 252]), (257, 8, 8, [3, 4])])
 def test_background_mode_matches_oracle(gpu, count, ny, nx, alphabet):
     import particleholography as ph
 
