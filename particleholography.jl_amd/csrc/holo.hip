@@ -1277,11 +1277,11 @@ static int link_rectangles(const int32_t* rects, const int64_t* offsets, int nsl
   link_finalize(t);
   for (size_t i = 0; i < t.size(); ++i) {
     int64_t* o = rows + 6 * i;
-    o[0] = t.x0[i];
-    o[1] = t.y0[i];
+    o[0] = t.x0(i);
+    o[1] = t.y0(i);
     o[2] = t.z0[i];
-    o[3] = t.x1[i];
-    o[4] = t.y1[i];
+    o[3] = t.x1(i);
+    o[4] = t.y1(i);
     o[5] = t.z1[i];
     born[i] = t.born[i];
     alive[i] = t.dead[i] ? 0 : 1;
