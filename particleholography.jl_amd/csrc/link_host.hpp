@@ -235,10 +235,20 @@ inline int64_t link_first_overlap(const LinkTable& t, LinkGrid* grid, uint32_t k
       if (k2 != k && !t.dead[k2] && t.overlap3d(k, k2)) return k2;
     return -1;
   }
+  // merged rows die as the passes go; they leave the cells as the scan passes them (dead is
+  // final, and nothing else walks the grid during the scan)
   for (int64_t b = grid->cy(t.y0(k)); b <= grid->cy(t.y1(k)); ++b)
-    for (int64_t a = grid->cx(t.x0(k)); a <= grid->cx(t.x1(k)); ++a)
-      for (uint32_t k2 : grid->cells[b * grid->nx + a])
-        if (k2 != k && (best < 0 || k2 < best) && !t.dead[k2] && t.overlap3d(k, k2)) best = k2;
+    for (int64_t a = grid->cx(t.x0(k)); a <= grid->cx(t.x1(k)); ++a) {
+      std::vector<uint32_t>& c = grid->cells[b * grid->nx + a];
+      size_t w = 0;
+      for (size_t q = 0; q < c.size(); ++q) {
+        const uint32_t k2 = c[q];
+        if (t.dead[k2]) continue;
+        c[w++] = k2;
+        if (k2 != k && (best < 0 || k2 < best) && t.overlap3d(k, k2)) best = k2;
+      }
+      c.resize(w);
+    }
   return best;
 }
 
