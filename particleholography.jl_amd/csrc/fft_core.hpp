@@ -272,6 +272,22 @@ template <int PS>
 __device__ __forceinline__ int lds_pad(int n) { return n + (n >> PS); }
 __host__ __device__ constexpr int lds_padded_len(int L, int PS) { return L + (L >> PS) + 1; }
 
+// Exchange ordering between the LDS writes and reads of one stage. A line owned by the lanes of
+// a single wavefront (64 % T == 0 with T threads per line, lines packed in thread order) needs
+// no workgroup barrier: LDS instructions of a wave execute in issue order, so a compiler fence
+// at wavefront scope keeps the reads after the writes. Lines that span waves use the workgroup
+// barrier.
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham stage: E/R butterflies per thread, outputs to LDS unless this is the last.
 template <int R, int E, int DIR, int PS>
 __device__ __forceinline__ void fft_stage(float2 (&v)[E], const int t, const int T,
@@ -401,7 +417,7 @@ struct CPlan {
   static constexpr int T = L / E;
 };
 
-template <int R, int E, int DIR, int T, int P, int OFF, bool LAST, int PS>
+template <int R, int E, int DIR, int T, int P, int OFF, bool LAST, int PS, bool WAVE>
 __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const float2* __restrict__ tw,
                                              float2* __restrict__ lds, const int es) {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
@@ -431,16 +447,16 @@ __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const 
     for (int m = 0; m < R; ++m) v[q + Q * m] = x[m];
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    lds_sync<WAVE>();  // every thread of the line has read the previous stage's LDS contents
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
       for (int m = 0; m < R; ++m) lds[lds_pad<PS>(base[q] + m * P) * es] = v[q + Q * m];
-    __syncthreads();
+    lds_sync<WAVE>();
   }
 }
 
-template <int E, int DIR, int L, int PS, int S = 0>
+template <int E, int DIR, int L, int PS, bool WAVE = false, int S = 0>
 __device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__ lds, const int es,
                                             const int t, const float2* __restrict__ tw) {
   constexpr CPlanData d = CPlan<L, E>::d;
@@ -450,26 +466,32 @@ __device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = lds[lds_pad<PS>(t + T * e) * es];
     }
-    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns, PS>(v, t, tw, lds, es);
-    fft_line_ct<E, DIR, L, PS, S + 1>(v, lds, es, t, tw);
+    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns, PS, WAVE>(v, t, tw, lds, es);
+    fft_line_ct<E, DIR, L, PS, WAVE, S + 1>(v, lds, es, t, tw);
   }
 }
 
-// Line FFT with a compile-time plan when L != 0, else the runtime plan `pl`.
-template <int E, int DIR, int L, int PS>
+// Line FFT with a compile-time plan when L != 0, else the runtime plan `pl`. WAVE: the line
+// lies within one wavefront (wave-scope ordering instead of workgroup barriers).
+template <int E, int DIR, int L, int PS, bool WAVE = false>
 __device__ __forceinline__ void fft_run(float2 (&v)[E], float2* __restrict__ lds, const int es,
                                         const int t, const LinePlan& pl, const float2* __restrict__ tw) {
   if constexpr (L != 0) {
-    fft_line_ct<E, DIR, L, PS>(v, lds, es, t, tw);
+    fft_line_ct<E, DIR, L, PS, WAVE>(v, lds, es, t, tw);
   } else {
     fft_line<E, DIR, PS>(v, lds, es, t, pl, tw);
   }
 }
+// Rows with a compile-time plan whose T = L/E threads divide a wavefront: every line lies in
+// one wave (row workgroups hold whole lines in thread order).
+template <int E, int L>
+constexpr bool kWaveRow = L != 0 && (64 % (L / E)) == 0;
+
 // A row line (stride 1 in LDS) and a column line of a tile of W interleaved columns.
 template <int E, int DIR, int L>
 __device__ __forceinline__ void fft_row(float2 (&v)[E], float2* __restrict__ line, const int t,
                                         const LinePlan& pl, const float2* __restrict__ tw) {
-  fft_run<E, DIR, L, kRowPad>(v, line, 1, t, pl, tw);
+  fft_run<E, DIR, L, kRowPad, kWaveRow<E, L>>(v, line, 1, t, pl, tw);
 }
 template <int E, int DIR, int L>
 __device__ __forceinline__ void fft_col(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,

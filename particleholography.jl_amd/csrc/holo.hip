@@ -445,6 +445,12 @@ int forward_spectrum(const Geom& g, int Ex, int Ey, const void* src, int src_rea
   return launch_cols_fft(g, Ey, -1, F, 0, F, 0, 1.0f, 1, st);
 }
 
+// 1/(nx*ny)^2: the projection pass takes the minimum of unscaled |IFFT|^2 and scales it once.
+float proj_scale(int nx, int ny) {
+  const double n = static_cast<double>(nx) * ny;
+  return static_cast<float>(1.0 / (n * n));
+}
+
 struct ReconPlan {
   int m;        // slices per batch
   int spg_c;    // slices per column workgroup
@@ -500,6 +506,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     prof_end(h, st, pe, HOLO_PASS_FWD_FFT, 1);
   }
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
+  const float sc2 = proj_scale(nx, ny);
   const size_t clds = col_lds(g), rlds = row_lds(g);
   for (int b0 = 0; b0 < slices; b0 += rp.m) {
     const int nb = std::min(rp.m, slices - b0);
@@ -550,7 +557,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     constexpr int L = decltype(lc)::value;
       if (kind == 0) {
         allow_lds(k_recon_rows_proj<E, L>, rlds);
-        hipLaunchKernelGGL((k_recon_rows_proj<E, L>), row_grid(g, ry), row_block(g), rlds, st, jt, plane, rp.spg_r, sc, g);
+        hipLaunchKernelGGL((k_recon_rows_proj<E, L>), row_grid(g, ry), row_block(g), rlds, st, jt, plane, rp.spg_r, sc2, g);
       } else {
         StoreArgs sa{};
         sa.G = G;
@@ -599,7 +606,7 @@ int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan
                  double dx, const JobTable& jt, int maxnb, hipStream_t st, bool do_a = true,
                  bool do_b = true) {
   const size_t plane = static_cast<size_t>(g.nx) * g.ny;
-  const float sc = static_cast<float>(1.0 / (static_cast<double>(g.nx) * g.ny));
+  const float sc2 = proj_scale(g.nx, g.ny);
   long long slices = 0;
   for (int i = 0; i < jt.n; ++i) slices += jt.d[i].nbatch;
   if (do_a) {
@@ -631,7 +638,7 @@ int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan
       allow_lds(k_recon_rows_proj<E, L>, rlds);
       dim3 grid = row_grid(g, (maxnb + rp.spg_r - 1) / rp.spg_r);
       grid.z = jt.n;
-      hipLaunchKernelGGL((k_recon_rows_proj<E, L>), grid, row_block(g), rlds, st, jt, plane, rp.spg_r, sc, g);
+      hipLaunchKernelGGL((k_recon_rows_proj<E, L>), grid, row_block(g), rlds, st, jt, plane, rp.spg_r, sc2, g);
     });
     if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
     HOLO_TRY(check_launch());

@@ -44,6 +44,17 @@ __device__ __forceinline__ double axis_a(int n, int N, double wavlen, double dx)
   return t * t;
 }
 
+// The pass kernels evaluate the same axis term with one product by s = wavlen / N / dx (computed
+// once per thread): t = (ic - N/2) * s differs from the reference's three-rounding form by at
+// most two FP64 ulps, a phase change below 1e-10 cycles at |z| = 2e5 um. The two FP64
+// divisions per element of the reference form were a fifth of pass A's instructions.
+__device__ __forceinline__ double axis_scale(int N, double wavlen, double dx) { return wavlen / N / dx; }
+__device__ __forceinline__ double axis_a_s(int n, int N, double s) {
+  const int ic = (n + (N >> 1)) % N;
+  const double t = (static_cast<double>(ic) - 0.5 * N) * s;
+  return t * t;
+}
+
 // exp(2 pi i u) for u in cycles, with u wrapped to [-1/2, 1/2] in FP64 before the FP32
 // sincospi. Evanescent components (d < 0) are returned as 0.
 __device__ __forceinline__ float2 cis_cycles(double u) {
@@ -169,6 +180,15 @@ __global__ void k_lpf_super_gaussian(int n, double sigma, double dx, float* __re
 // STAGE false leaves the twiddle copy to the kernel, which then issues its global loads
 // first: the copy's LDS writes wait for the table, and a single-plane launch (two waves per
 // SIMD) cannot hide that latency behind other waves.
+// The twiddle table is staged by every thread of the workgroup. A row FFT whose lines lie in one
+// wavefront orders its LDS exchanges at wavefront scope only, so it needs one workgroup barrier
+// after the staging before a stage reads another wave's part of the table; other plans get that
+// barrier from their first exchange.
+template <int E, int L>
+__device__ __forceinline__ void twiddles_visible() {
+  if constexpr (kWaveRow<E, L>) __syncthreads();
+}
+
 struct RowCtx {
   int t, w, j;
   int jl;  // j clamped into the plane: loads read row jl unconditionally (see below)
@@ -187,7 +207,10 @@ __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
   c.jl = c.ok ? c.j : g.ny - 1;
   c.line = lds + c.w * lds_padded_len(g.nx, kRowPad);
   float2* tw = lds + g.rows * lds_padded_len(g.nx, kRowPad);
-  if (STAGE) stage_twiddles(tw, g.twx, g.px.ntw);
+  if (STAGE) {
+    stage_twiddles(tw, g.twx, g.px.ntw);
+    twiddles_visible<E, L>();
+  }
   c.tw = tw;
   return c;
 }
@@ -247,6 +270,7 @@ __global__ __launch_bounds__(512) void k_rows_fft(const void* __restrict__ in, s
     }
   }
   stage_twiddles(c.tw, g.twx, g.px.ntw);
+  twiddles_visible<E, L>();
   fft_row<E, DIR, L>(v, c.line, c.t, g.px, c.tw);
   if (c.ok) {
 #pragma unroll
@@ -350,11 +374,11 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
     float2* __restrict__ G = jd.G;
     float2 r[E];
     {
-      const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
+      const double ax = axis_a_s(ci, g.nx, axis_scale(g.nx, a.gp.wavlen, a.gp.dx));
+      const double sy = axis_scale(g.ny, a.gp.wavlen, a.gp.dx);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-#pragma clang fp contract(off)
-        const double d = (1.0 - ax) - axis_a(c.t + T * e, g.ny, a.gp.wavlen, a.gp.dx);
+        const double d = (1.0 - ax) - axis_a_s(c.t + T * e, g.ny, sy);
         if (d >= 0.0) {
           const double sq = sqrt(d);
           f[e] = cmul(f[e], cis_cycles(zl * sq));
@@ -437,10 +461,14 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, size_t n, unsigned int 
 }
 
 // Projection epilogue (A6): proj = min over slices of |psi_k|^2, psi_k = IFFT/(nx*ny). The
-// minimum follows CUDA.jl's device min for Float32 (fminf: a NaN operand is ignored).
+// minimum follows CUDA.jl's device min for Float32 (fminf: a NaN operand is ignored). The
+// workgroup keeps the minimum of the unscaled |IFFT|^2 as an unsigned bit pattern (the order of
+// non-negative floats, NaN above them, as in the merge below) and scales it once by
+// sc2 = 1/(nx*ny)^2 before the merge; for power-of-two sizes the scaling is exact and the result
+// equals the per-slice scaled form bit for bit.
 template <int E, int L>
 __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gstride, int spg,
-                                                         float sc, Geom g) {
+                                                         float sc2, Geom g) {
   extern __shared__ float2 lds[];
   const JobDesc& jd = jt.d[blockIdx.z];
   const int nbatch = jd.nbatch;
@@ -449,27 +477,24 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
   float* __restrict__ proj = jd.proj;
   const RowCtx c = row_ctx<E, L>(g, lds);
   const int T = line_threads<E, L>(g.px);
-  const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
+  const size_t rowoff = static_cast<size_t>(c.jl) * g.nx;
   const int k0 = blockIdx.y * spg;
   const int k1 = min(k0 + spg, nbatch);
-  float mn[E];
+  unsigned mn[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) mn[e] = __uint_as_float(0x7fffffffu);
+  for (int e = 0; e < E; ++e) mn[e] = 0x7fffffffu;
   for (int k = k0; k < k1; ++k) {
     float2 v[E];
     const float2* __restrict__ src = G + static_cast<size_t>(k) * gstride + rowoff;
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = c.ok ? src[c.t + T * e] : make_float2(0.f, 0.f);
+    for (int e = 0; e < E; ++e) v[e] = src[c.t + T * e];
     fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float a = v[e].x * sc, b = v[e].y * sc;
-      mn[e] = fminf(mn[e], a * a + b * b);
-    }
+    for (int e = 0; e < E; ++e) mn[e] = min(mn[e], __float_as_uint(v[e].x * v[e].x + v[e].y * v[e].y));
   }
   if (c.ok && k1 > k0) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) atomic_min_nonneg(proj + rowoff + c.t + T * e, mn[e]);
+    for (int e = 0; e < E; ++e) atomic_min_nonneg(proj + rowoff + c.t + T * e, __uint_as_float(mn[e]) * sc2);
   }
 }
 
@@ -642,11 +667,11 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
   stage_twiddles(c.tw, g.twy, g.py.ntw);
   fft_col<E, -1, L>(v, c.col, g.W, c.t, g.py, c.tw);
   if constexpr (HMODE == 0) {
-    const double ax = axis_a(ci, g.nx, a.gp.wavlen, a.gp.dx);
+    const double ax = axis_a_s(ci, g.nx, axis_scale(g.nx, a.gp.wavlen, a.gp.dx));
+    const double sy = axis_scale(g.ny, a.gp.wavlen, a.gp.dx);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-#pragma clang fp contract(off)
-      const double d = (1.0 - ax) - axis_a(c.t + T * e, g.ny, a.gp.wavlen, a.gp.dx);
+      const double d = (1.0 - ax) - axis_a_s(c.t + T * e, g.ny, sy);
       const float2 h = d >= 0.0 ? cis_cycles(a.gp.zl0 * sqrt(d)) : make_float2(0.f, 0.f);
       v[e] = cmul(v[e], h);
     }
@@ -690,6 +715,7 @@ __global__ __launch_bounds__(512) void k_gs_rows(GsTable gt, float sc, Geom g) {
     iv[e] = I[rowoff + c.t + T * e];
   }
   stage_twiddles(c.tw, g.twx, g.px.ntw);
+  twiddles_visible<E, L>();
   fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
