@@ -110,6 +110,21 @@ int holo_reconst_xyprojection_holo(holo_handle h, void* stream, int nx, int ny,
                                    const float* holo, const holo_tf* front,
                                    const holo_tf* dz, int slices, float* proj);
 
+/* Allocation-free variant of holo_reconst_xyprojection. No counterpart in the reference (whose
+ * functions allocate through CUDA.jl's pool); it is for callers that cannot allocate device
+ * memory: code under graph capture, tight serving loops, harnesses that forbid hipMalloc. Same
+ * arguments, results within FP32 rounding of holo_reconst_xyprojection's. It calls no hipMalloc
+ * or hipFree: its workspace is a static 128 MiB device array (one per GPU, held from the
+ * library's load until the process ends) whose address holo_create looks up; FFT twiddle
+ * tables not cached on the handle are copied into it. Sizes whose spectrum plus one slice
+ * (2 planes, 3 for array transfers, of nx*ny*8 bytes) exceed that array return
+ * HOLO_ERR_BUDGET; smaller sizes run in rounds that fit it. Every call on one GPU shares the
+ * array: calls that overlap on one GPU (other streams, other handles, other threads) race and
+ * must be serialized by the caller. */
+int holo_reconst_xyprojection_noalloc(holo_handle h, void* stream, int nx, int ny,
+                                      const void* wavefront, const holo_tf* front,
+                                      const holo_tf* dz, int slices, float* proj);
+
 /* ---- A7 / A8 / A9: cu_get_reconst_vol, cu_get_reconst_complex_vol,
  *      cu_get_reconst_vol_and_xyprojection (holofunc.jl:184-200, 216-232, 300-321) -------- */
 /* vol (nx*ny*slices of out_dtype). If proj != NULL (A9, Float32 nx*ny) it receives the min over

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <tuple>
@@ -121,6 +122,11 @@ struct holo_context {
   size_t batch_ws_limit = 0;                      // batched calls' workspace cap (0 = default)
   hipStream_t cap = nullptr;
   double2* partials = nullptr;  // 256 partial sums for cu_2d_pad's mean
+  // The allocation-free entry point: this device's address of the static scratch array, and
+  // the host halves of the plans it built there (kept so their twiddle tables outlive the
+  // asynchronous copies into the scratch).
+  unsigned char* scratch = nullptr;
+  std::map<int, std::pair<LinePlan, std::vector<float2>>> host_plans;
   // A GS graph reads and writes only the workspace (field, transfer tables, copies of the two
   // intensities), so its key holds no caller pointer and one graph serves every call of a size.
   struct GsKey {
@@ -189,29 +195,18 @@ void prof_collect(holo_context* h) {
   h->prof_pending.clear();
 }
 
-// Plan of a length with E elements per thread (E_req 0: the default E of the length).
-int get_plan(holo_context* h, int L, const Plan1D** out, int E_req = 0) {
-  const int E = E_req > 0 ? E_req : choose_E(L);
-  if (E == 0)
-    return fail(HOLO_ERR_SIZE, "FFT length " + std::to_string(L) +
-                                   " not supported (need 2^a, 3*2^a or 5*2^a, <= 4096)");
-  const int key = (L << 8) | E;
-  auto it = h->plans.find(key);
-  if (it != h->plans.end()) {
-    *out = &it->second;
-    return HOLO_OK;
-  }
-  Plan1D p;
-  p.E = E;
-  p.lp.L = L;
-  p.lp.T = L / E;
-  p.lp.ns = 0;
+// Host half of a plan: the radix schedule of the device's compile-time planner and the
+// FP64-computed twiddle table, rounded to float.
+int build_plan(int L, int E, LinePlan* lp, std::vector<float2>* tw) {
+  lp->L = L;
+  lp->T = L / E;
+  lp->ns = 0;
   // the schedule of the device's compile-time plans (fft_core.hpp: make_cplan)
   const CPlanData c = make_cplan(L, E);
   if (!c.complete) return fail(HOLO_ERR_SIZE, "no radix plan for length " + std::to_string(L));
-  std::vector<float2> tw;
+  tw->clear();
   for (int s = 0; s < c.ns; ++s) {
-    StagePlan& sp = p.lp.st[p.lp.ns++];
+    StagePlan& sp = lp->st[lp->ns++];
     const int R = c.R[s], span = c.P[s];
     sp.R = R;
     sp.p = span;
@@ -223,17 +218,67 @@ int get_plan(holo_context* h, int L, const Plan1D** out, int E_req = 0) {
         for (int k = 0; k < span; ++k) {
           const long long mk = (static_cast<long long>(m) * k) % PR;
           const double a = -2.0 * M_PI * static_cast<double>(mk) / static_cast<double>(PR);
-          tw.push_back(make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a))));
+          tw->push_back(make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a))));
         }
     }
   }
-  p.lp.ntw = c.ntw;
-  if (static_cast<int>(tw.size()) != c.ntw) return fail(HOLO_ERR_SIZE, "internal: twiddle table size");
-  if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+  lp->ntw = c.ntw;
+  if (static_cast<int>(tw->size()) != c.ntw) return fail(HOLO_ERR_SIZE, "internal: twiddle table size");
+  if (tw->empty()) tw->push_back(make_float2(1.f, 0.f));
+  return HOLO_OK;
+}
+
+int plan_E(int L, int E_req) {
+  const int E = E_req > 0 ? E_req : choose_E(L);
+  if (E == 0)
+    fail(HOLO_ERR_SIZE, "FFT length " + std::to_string(L) + " not supported (need 2^a, 3*2^a or 5*2^a, <= 4096)");
+  return E;
+}
+
+// Plan of a length with E elements per thread (E_req 0: the default E of the length).
+int get_plan(holo_context* h, int L, const Plan1D** out, int E_req = 0) {
+  const int E = plan_E(L, E_req);
+  if (E == 0) return HOLO_ERR_SIZE;
+  const int key = (L << 8) | E;
+  auto it = h->plans.find(key);
+  if (it != h->plans.end()) {
+    *out = &it->second;
+    return HOLO_OK;
+  }
+  Plan1D p;
+  p.E = E;
+  std::vector<float2> tw;
+  HOLO_TRY(build_plan(L, E, &p.lp, &tw));
   HOLO_HIP(hipMalloc(&p.dtw, tw.size() * sizeof(float2)));
   HOLO_HIP(hipMemcpy(p.dtw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
   auto ins = h->plans.emplace(key, p);
   *out = &ins.first->second;
+  return HOLO_OK;
+}
+
+// A plan without device allocation: the handle's cached plan when it has one, otherwise the
+// host plan (built once, kept on the handle) with its twiddle table copied into `slot`, a
+// region of the static scratch array, on `st`.
+int get_plan_noalloc(holo_context* h, int L, int E_req, float2* slot, hipStream_t st, Plan1D* out) {
+  const int E = plan_E(L, E_req);
+  if (E == 0) return HOLO_ERR_SIZE;
+  const int key = (L << 8) | E;
+  auto it = h->plans.find(key);
+  if (it != h->plans.end()) {
+    *out = it->second;
+    return HOLO_OK;
+  }
+  auto hp = h->host_plans.find(key);
+  if (hp == h->host_plans.end()) {
+    std::pair<LinePlan, std::vector<float2>> e;
+    HOLO_TRY(build_plan(L, E, &e.first, &e.second));
+    hp = h->host_plans.emplace(key, std::move(e)).first;
+  }
+  HOLO_HIP(hipMemcpyAsync(slot, hp->second.second.data(), hp->second.second.size() * sizeof(float2),
+                          hipMemcpyHostToDevice, st));
+  out->lp = hp->second.first;
+  out->E = E;
+  out->dtw = slot;
   return HOLO_OK;
 }
 
@@ -300,8 +345,16 @@ Tuning plane_tiles(const holo_context* h, int nx, int ny) {
 
 // row_threads: target threads per row-pass workgroup when the row count is not tuned (512 for
 // the reconstruction's pass B, which loops over slices; 256 elsewhere).
+// Where make_geom takes its plans from when it must not allocate: the twiddle tables of plans
+// the handle has not cached go to two fixed slots (x, y) of the static scratch array.
+struct NoAllocPlans {
+  float2* slot[2];
+  hipStream_t st;
+  Plan1D px, py;
+};
+
 int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int row_threads = 256,
-              const Tuning* tiles = nullptr) {
+              const Tuning* tiles = nullptr, NoAllocPlans* na = nullptr) {
   if (nx < 1 || ny < 1) return fail(HOLO_ERR_ARG, "sizes must be positive");
   const Plan1D *px, *py;
   // passes may run with fewer elements per thread (more threads per line, fewer registers)
@@ -310,8 +363,15 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
     return (want > 0 && L % want == 0 && L / want <= 512 && want < choose_E(L)) ? want : 0;
   };
   const Tuning tn = tiles ? *tiles : eff_tuning(h, nx, ny);
-  HOLO_TRY(get_plan(h, nx, &px, fewer(tn.erow, nx)));
-  HOLO_TRY(get_plan(h, ny, &py, fewer(tn.ecol ? tn.ecol : h->col_e, ny)));
+  if (na) {
+    HOLO_TRY(get_plan_noalloc(h, nx, fewer(tn.erow, nx), na->slot[0], na->st, &na->px));
+    HOLO_TRY(get_plan_noalloc(h, ny, fewer(tn.ecol ? tn.ecol : h->col_e, ny), na->slot[1], na->st, &na->py));
+    px = &na->px;
+    py = &na->py;
+  } else {
+    HOLO_TRY(get_plan(h, nx, &px, fewer(tn.erow, nx)));
+    HOLO_TRY(get_plan(h, ny, &py, fewer(tn.ecol ? tn.ecol : h->col_e, ny)));
+  }
   std::memset(g, 0, sizeof(Geom));
   g->nx = nx;
   g->ny = ny;
@@ -473,10 +533,18 @@ ReconPlan recon_plan(holo_context* h, const Geom& g, int slices) {
 
 // Core of A6-A9 and the padded variant. kind: 0 projection (fminf, A6); 1 store (A7-A9).
 // The workspace holds [reserve bytes | F | G (batch of slices) | running spectrum].
+// Fixed workspace of the allocation-free entry point: `buf` (cap bytes) replaces the handle's
+// workspace, rounds shrink to fit it, plans come from `plans`, and no pass is timed.
+struct FixedWs {
+  void* buf;
+  size_t cap;
+  NoAllocPlans* plans;
+};
+
 int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src, int src_real,
                 const holo_tf* front, const holo_tf* dz, int slices, int kind,
                 int out_dtype, void* vol, float* proj, int ox, int oy, int onx, int ony,
-                size_t reserve = 0) {
+                size_t reserve = 0, const FixedWs* fixed = nullptr) {
   HOLO_TRY(validate_transfer(front, "front"));
   HOLO_TRY(validate_transfer(dz, "dz"));
   if (front->generated != dz->generated)
@@ -489,19 +557,34 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     return fail(HOLO_ERR_ARG, "a cropped output window is an N0f8 volume");
   Geom g;
   int Ex, Ey;
-  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 512));
-  const ReconPlan rp = recon_plan(h, g, slices);
+  HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 512, nullptr, fixed ? fixed->plans : nullptr));
+  ReconPlan rp = recon_plan(h, g, slices);
   const size_t plane = static_cast<size_t>(nx) * ny;
   const bool arrays = !front->generated;
-  const size_t need = reserve + plane * sizeof(float2) * (1 + static_cast<size_t>(rp.m) + (arrays ? 1 : 0));
-  HOLO_TRY(ensure_ws(h, need));
-  float2* F = reinterpret_cast<float2*>(static_cast<char*>(h->ws) + reserve);
+  void* ws = nullptr;
+  if (fixed) {
+    // the spectrum (and the running spectrum of array transfers) plus at least one slice
+    const size_t planes = fixed->cap / (plane * sizeof(float2));
+    const size_t other = 1 + (arrays ? 1 : 0);
+    if (planes < other + 1)
+      return fail(HOLO_ERR_BUDGET, std::to_string(nx) + " x " + std::to_string(ny) + " needs " +
+                                       std::to_string((other + 1) * plane * sizeof(float2)) +
+                                       " bytes of scratch; the allocation-free entry point holds " +
+                                       std::to_string(fixed->cap));
+    rp.m = static_cast<int>(std::min<size_t>(rp.m, planes - other));
+    ws = fixed->buf;
+  } else {
+    const size_t need = reserve + plane * sizeof(float2) * (1 + static_cast<size_t>(rp.m) + (arrays ? 1 : 0));
+    HOLO_TRY(ensure_ws(h, need));
+    ws = h->ws;
+  }
+  float2* F = reinterpret_cast<float2*>(static_cast<char*>(ws) + reserve);
   float2* G = F + plane;
   float2* run = G + plane * rp.m;
   const size_t oplane = static_cast<size_t>(onx) * ony;
   if (proj) HOLO_TRY(launch_fill(proj, oplane, 0x7fffffffu, st));
   {
-    hipEvent_t pe = prof_begin(h, st);
+    hipEvent_t pe = fixed ? nullptr : prof_begin(h, st);
     HOLO_TRY(forward_spectrum(g, Ex, Ey, src, src_real, F, st));
     prof_end(h, st, pe, HOLO_PASS_FWD_FFT, 1);
   }
@@ -527,7 +610,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
       ra.run = run;
       ra.first = b0 == 0;
     }
-    hipEvent_t pa = prof_begin(h, st);
+    hipEvent_t pa = fixed ? nullptr : prof_begin(h, st);
     JobTable jt{};
     jt.n = 1;
     jt.d[0].F = F;
@@ -551,7 +634,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     HOLO_TRY(check_launch());
     prof_end(h, st, pa, HOLO_PASS_RECON_COLS, nb);
     const int ry = (nb + rp.spg_r - 1) / rp.spg_r;
-    hipEvent_t pb = prof_begin(h, st);
+    hipEvent_t pb = fixed ? nullptr : prof_begin(h, st);
     ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
       constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
@@ -784,6 +867,33 @@ int gs_transfers(const Geom& g, const holo_tf* fwd, const holo_tf* bwd, float2* 
 
 }  // namespace
 
+// Allocation-free reconstruction: a static device array serves as its workspace. Each device
+// holds its own copy (kScratchBytes, from the library's load on that device until the process
+// ends); the host-side symbol is not a device address, so the address comes from
+// hipGetSymbolAddress, once per device with that device current (holo_create), and only that
+// address is ever passed to kernels or copies. Layout: [x twiddles | y twiddles | workspace].
+constexpr size_t kScratchBytes = 128ull << 20;
+constexpr size_t kScratchTwBytes = 4096 * sizeof(float2);  // one axis's table: < 4096 entries
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+int scratch_address(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return fail(HOLO_ERR_ARG, "device index beyond the scratch table");
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    HOLO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch[device];
+  return HOLO_OK;
+}
+}  // namespace
+
 // =========================================================================================
 // C ABI
 // =========================================================================================
@@ -843,6 +953,12 @@ int holo_create(holo_handle* out, int device) {
     (void)hipStreamDestroy(h->cap);
     delete h;
     return fail(HOLO_ERR_NOMEM, "partial-sum buffer allocation failed");
+  }
+  if (const int r = scratch_address(device, &h->scratch); r != HOLO_OK) {
+    (void)hipFree(h->partials);
+    (void)hipStreamDestroy(h->cap);
+    delete h;
+    return r;
   }
   *out = h;
   return HOLO_OK;
@@ -1320,6 +1436,20 @@ int holo_reconst_xyprojection_holo(holo_handle h, void* stream, int nx, int ny, 
   if (!h || !proj) return fail(HOLO_ERR_ARG, "NULL handle or output");
   return reconstruct(h, static_cast<hipStream_t>(stream), nx, ny, holo, 1, front, dz, slices, 0,
                      HOLO_F32, nullptr, proj, 0, 0, nx, ny);
+}
+
+int holo_reconst_xyprojection_noalloc(holo_handle h, void* stream, int nx, int ny, const void* wavefront,
+                                      const holo_tf* front, const holo_tf* dz, int slices, float* proj) {
+  if (!h || !proj) return fail(HOLO_ERR_ARG, "NULL handle or output");
+  if (!h->scratch) return fail(HOLO_ERR_ARG, "handle has no scratch address");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  NoAllocPlans na{};
+  na.slot[0] = reinterpret_cast<float2*>(h->scratch);
+  na.slot[1] = reinterpret_cast<float2*>(h->scratch + kScratchTwBytes);
+  na.st = st;
+  FixedWs fw{h->scratch + 2 * kScratchTwBytes, kScratchBytes - 2 * kScratchTwBytes, &na};
+  return reconstruct(h, st, nx, ny, wavefront, 0, front, dz, slices, 0, HOLO_F32, nullptr, proj, 0, 0, nx,
+                     ny, 0, &fw);
 }
 
 int holo_reconst_vol(holo_handle h, void* stream, int nx, int ny, const void* wavefront,

@@ -66,6 +66,7 @@ SIGNATURES = {
     "holo_gabor_wavefront": [_vp, _vp, _i, _i, _vp, _vp],
     "holo_reconst_xyprojection": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _vp],
     "holo_reconst_xyprojection_holo": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _vp],
+    "holo_reconst_xyprojection_noalloc": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _vp],
     "holo_reconst_vol": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _i, _vp, _vp],
     "holo_2d_pad": [_vp, _vp, _i, _i, _vp, _vp],
     "holo_reconst_vol_and_xyprojection_padded": [_vp, _vp, _i, _i, _vp, _tfp, _tfp, _i, _vp, _vp],
