@@ -73,9 +73,11 @@ int holo_destroy(holo_handle h);
 const char* holo_last_error(void);
 int holo_version(void);
 int holo_supported_length(int n);
-/* Tuning knobs (0 = library default): slices per batch, slices per workgroup in the column
- * pass and in the row pass, columns per workgroup (power of two), rows per workgroup. */
-int holo_set_tuning(holo_handle h, int batch, int spg_cols, int spg_rows, int cols_per_wg,
+/* Tuning knobs (0 = library default): slices per batch (round), slices per workgroup of the
+ * reconstruction pass that generates H_k (spg_gen; the row pass of projections, the column pass
+ * of volumes) and of the pass that forms the output (spg_out), columns per workgroup of column
+ * passes (power of two), rows per workgroup of row passes. */
+int holo_set_tuning(holo_handle h, int batch, int spg_gen, int spg_out, int cols_per_wg,
                     int rows_per_wg);
 
 /* ---- A2: cu_transfer_sqrt_arr (holofunc.jl:8-15, 30-36) --------------------------------- */
@@ -283,9 +285,9 @@ int holo_set_workspace_limit(holo_handle h, size_t bytes);
 int holo_workspace_size(holo_handle h, size_t* bytes);
 /* Per-size tile choices (0 = default): slices per batch, slices per column / row workgroup,
  * columns per workgroup, rows per workgroup. The global holo_set_tuning overrides them. */
-int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_cols, int spg_rows,
+int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_gen, int spg_out,
                              int cols_per_wg, int rows_per_wg);
-/* The effective choices for a size (defaults resolved) as {batch, spg_cols, spg_rows, cols, rows}. */
+/* The effective choices for a size (defaults resolved) as {batch, spg_gen, spg_out, cols, rows}. */
 int holo_get_tuning_for_size(holo_handle h, int nx, int ny, int slices, int* out5);
 /* Times candidate tile configurations of both reconstruction passes on the device (HIP events,
  * synchronous) and keeps the fastest for this size; the tiles in force before the call stay

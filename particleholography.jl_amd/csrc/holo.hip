@@ -56,7 +56,7 @@ struct Plan1D {
 };
 
 struct Tuning {
-  int batch = 0, spg_cols = 0, spg_rows = 0, cols = 0, rows = 0;
+  int batch = 0, spg_gen = 0, spg_out = 0, cols = 0, rows = 0;
   int erow = 0, ecol = 0;  // elements per thread of row / column passes (0: the length's default)
 };
 
@@ -308,8 +308,8 @@ Tuning eff_tuning(const holo_context* h, int nx, int ny) {
   auto it = h->size_tune.find({nx, ny});
   if (it != h->size_tune.end()) t = it->second;
   if (h->tune.batch > 0) t.batch = h->tune.batch;
-  if (h->tune.spg_cols > 0) t.spg_cols = h->tune.spg_cols;
-  if (h->tune.spg_rows > 0) t.spg_rows = h->tune.spg_rows;
+  if (h->tune.spg_gen > 0) t.spg_gen = h->tune.spg_gen;
+  if (h->tune.spg_out > 0) t.spg_out = h->tune.spg_out;
   if (h->tune.cols > 0) t.cols = h->tune.cols;
   if (h->tune.rows > 0) t.rows = h->tune.rows;
   return t;
@@ -513,8 +513,8 @@ float proj_scale(int nx, int ny) {
 
 struct ReconPlan {
   int m;        // slices per batch
-  int spg_c;    // slices per column workgroup
-  int spg_r;    // slices per row workgroup
+  int spg_gen;  // slices per workgroup of the pass that generates H_k
+  int spg_out;  // slices per workgroup of the pass that forms the output
 };
 
 ReconPlan recon_plan(holo_context* h, const Geom& g, int slices) {
@@ -524,15 +524,22 @@ ReconPlan recon_plan(holo_context* h, const Geom& g, int slices) {
   const Tuning tn = eff_tuning(h, g.nx, g.ny);
   int m = tn.batch > 0 ? tn.batch : static_cast<int>(std::max<size_t>(1, budget / plane));
   r.m = std::max(1, std::min(m, slices));
-  // Column workgroups advance up to 16 slices by the ComplexF32 rotation (drift <= 2e-6);
-  // row workgroups merge 8 slices per atomic min.
-  r.spg_c = tn.spg_cols > 0 ? tn.spg_cols : 16;
-  r.spg_r = tn.spg_rows > 0 ? tn.spg_rows : 8;
+  // Generating workgroups advance up to 16 slices by the ComplexF32 rotation (drift <= 2e-6);
+  // output workgroups merge 16 slices per atomic min (1024^2 x 1000: 3.52 ms against 3.58 ms
+  // with 8).
+  r.spg_gen = tn.spg_gen > 0 ? tn.spg_gen : 16;
+  r.spg_out = tn.spg_out > 0 ? tn.spg_out : 16;
   return r;
 }
 
 // Core of A6-A9 and the padded variant. kind: 0 projection (fminf, A6); 1 store (A7-A9).
 // The workspace holds [reserve bytes | F | G (batch of slices) | running spectrum].
+int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan& rp, double wavlen,
+                 double dx, const JobTable& jt, int maxnb, hipStream_t st, bool do_a = true,
+                 bool do_b = true, bool timed = true);
+int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp, double wavlen, double dx,
+                    const JobTable& jt, int maxnb, hipStream_t st);
+
 // Fixed workspace of the allocation-free entry point: `buf` (cap bytes) replaces the handle's
 // workspace, rounds shrink to fit it, plans come from `plans`, and no pass is timed.
 struct FixedWs {
@@ -588,6 +595,18 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     HOLO_TRY(forward_spectrum(g, Ex, Ey, src, src_real, F, st));
     prof_end(h, st, pe, HOLO_PASS_FWD_FFT, 1);
   }
+  if (kind == 0 && !arrays) {
+    // generated transfers: the projection pair of passes, one round at a time
+    for (int b0 = 0; b0 < slices; b0 += rp.m) {
+      JobTable jt{};
+      jt.n = 1;
+      jt.d[0] = JobDesc{F, G, proj, (front->z + static_cast<double>(b0) * dz->z) / front->wavlen,
+                        dz->z / front->wavlen, std::min(rp.m, slices - b0)};
+      HOLO_TRY(launch_round(h, g, Ex, Ey, rp, front->wavlen, front->dx, jt, jt.d[0].nbatch, st, true, true,
+                            fixed == nullptr));
+    }
+    return HOLO_OK;
+  }
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
   const float sc2 = proj_scale(nx, ny);
   const size_t clds = col_lds(g), rlds = row_lds(g);
@@ -598,7 +617,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     ra.G = G;
     ra.gstride = plane;
     ra.nbatch = nb;
-    ra.spg = rp.spg_c;
+    ra.spg = rp.spg_gen;
     if (!arrays) {
       ra.gp.wavlen = front->wavlen;
       ra.gp.dx = front->dx;
@@ -619,34 +638,33 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     jt.d[0].zl0 = ra.gp.zl0;
     jt.d[0].dzl = ra.gp.dzl;
     jt.d[0].nbatch = nb;
-    bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
-      constexpr int E = decltype(ec)::value;
-    constexpr int L = decltype(lc)::value;
-      if (!arrays) {
-        allow_lds(k_recon_cols<E, L, 0>, clds);
-        hipLaunchKernelGGL((k_recon_cols<E, L, 0>), col_grid(g, (nb + rp.spg_c - 1) / rp.spg_c), col_block(g), clds, st, ra, jt, g);
-      } else {
+    if (!arrays) {
+      HOLO_TRY(launch_gen_cols(h, g, Ey, rp, front->wavlen, front->dx, jt, nb, st));
+    } else {
+      bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
+        constexpr int E = decltype(ec)::value;
+        constexpr int L = decltype(lc)::value;
         allow_lds(k_recon_cols<E, L, 1>, clds);
         hipLaunchKernelGGL((k_recon_cols<E, L, 1>), col_grid(g, 1), col_block(g), clds, st, ra, jt, g);
-      }
-    });
-    if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
-    HOLO_TRY(check_launch());
+      });
+      if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
+      HOLO_TRY(check_launch());
+    }
     prof_end(h, st, pa, HOLO_PASS_RECON_COLS, nb);
-    const int ry = (nb + rp.spg_r - 1) / rp.spg_r;
+    const int ry = (nb + rp.spg_out - 1) / rp.spg_out;
     hipEvent_t pb = fixed ? nullptr : prof_begin(h, st);
-    ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
+    bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
       constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
       if (kind == 0) {
         allow_lds(k_recon_rows_proj<E, L>, rlds);
-        hipLaunchKernelGGL((k_recon_rows_proj<E, L>), row_grid(g, ry), row_block(g), rlds, st, jt, plane, rp.spg_r, sc2, g);
+        hipLaunchKernelGGL((k_recon_rows_proj<E, L>), row_grid(g, ry), row_block(g), rlds, st, jt, plane, rp.spg_out, sc2, g);
       } else {
         StoreArgs sa{};
         sa.G = G;
         sa.gstride = plane;
         sa.nbatch = nb;
-        sa.spg = rp.spg_r;
+        sa.spg = rp.spg_out;
         sa.vol = vol;
         sa.vstride = oplane;
         sa.slice0 = b0;
@@ -682,46 +700,54 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
   return HOLO_OK;
 }
 
-// One round of the generated-H reconstruction for every hologram in `jt` (one size): pass A
-// over (column tiles, slice groups, holograms), then pass B over (row tiles, slice groups,
-// holograms). `maxnb` is the largest per-hologram slice count of the round.
+// Pass A of a generated-H round: the column pass that generates H_k, over (column tiles, slice
+// groups, holograms).
+int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp, double wavlen, double dx,
+                    const JobTable& jt, int maxnb, hipStream_t st) {
+  const size_t plane = static_cast<size_t>(g.nx) * g.ny;
+  RecArgs ra{};
+  ra.gstride = plane;
+  ra.spg = rp.spg_gen;
+  ra.gp.wavlen = wavlen;
+  ra.gp.dx = dx;
+  const size_t lds = col_lds(g);
+  bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
+    constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
+    dim3 grid = col_grid(g, (maxnb + rp.spg_gen - 1) / rp.spg_gen);
+    grid.z = jt.n;
+    allow_lds(k_recon_cols<E, L, 0>, lds);
+    hipLaunchKernelGGL((k_recon_cols<E, L, 0>), grid, col_block(g), lds, st, ra, jt, g);
+  });
+  if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
+  return check_launch();
+}
+
+// One round of the generated-H projection reconstruction for every hologram in `jt` (one size):
+// pass A (the H-generating column pass) over (column tiles, slice groups, holograms), then pass B
+// (k_recon_rows_proj) over (row tiles, slice groups, holograms). `maxnb` is the largest
+// per-hologram slice count of the round.
 int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan& rp, double wavlen,
-                 double dx, const JobTable& jt, int maxnb, hipStream_t st, bool do_a = true,
-                 bool do_b = true) {
+                 double dx, const JobTable& jt, int maxnb, hipStream_t st, bool do_a, bool do_b, bool timed) {
   const size_t plane = static_cast<size_t>(g.nx) * g.ny;
   const float sc2 = proj_scale(g.nx, g.ny);
   long long slices = 0;
   for (int i = 0; i < jt.n; ++i) slices += jt.d[i].nbatch;
   if (do_a) {
-    RecArgs ra{};
-    ra.gstride = plane;
-    ra.spg = rp.spg_c;
-    ra.gp.wavlen = wavlen;
-    ra.gp.dx = dx;
-    const size_t clds = col_lds(g);
-    hipEvent_t pa = prof_begin(h, st);
-    bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
-      constexpr int E = decltype(ec)::value;
-      constexpr int L = decltype(lc)::value;
-      allow_lds(k_recon_cols<E, L, 0>, clds);
-      dim3 grid = col_grid(g, (maxnb + rp.spg_c - 1) / rp.spg_c);
-      grid.z = jt.n;
-      hipLaunchKernelGGL((k_recon_cols<E, L, 0>), grid, col_block(g), clds, st, ra, jt, g);
-    });
-    if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
-    HOLO_TRY(check_launch());
+    hipEvent_t pa = timed ? prof_begin(h, st) : nullptr;
+    HOLO_TRY(launch_gen_cols(h, g, Ey, rp, wavlen, dx, jt, maxnb, st));
     prof_end(h, st, pa, HOLO_PASS_RECON_COLS, slices);
   }
   if (do_b) {
     const size_t rlds = row_lds(g);
-    hipEvent_t pb = prof_begin(h, st);
+    hipEvent_t pb = timed ? prof_begin(h, st) : nullptr;
     bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
       constexpr int E = decltype(ec)::value;
       constexpr int L = decltype(lc)::value;
       allow_lds(k_recon_rows_proj<E, L>, rlds);
-      dim3 grid = row_grid(g, (maxnb + rp.spg_r - 1) / rp.spg_r);
+      dim3 grid = row_grid(g, (maxnb + rp.spg_out - 1) / rp.spg_out);
       grid.z = jt.n;
-      hipLaunchKernelGGL((k_recon_rows_proj<E, L>), grid, row_block(g), rlds, st, jt, plane, rp.spg_r, sc2, g);
+      hipLaunchKernelGGL((k_recon_rows_proj<E, L>), grid, row_block(g), rlds, st, jt, plane, rp.spg_out, sc2, g);
     });
     if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
     HOLO_TRY(check_launch());
@@ -927,7 +953,7 @@ int holo_create(holo_handle* out, int device) {
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
-      int nx, ny, batch, spg_cols, spg_rows, cols, rows, erow = 0, ecol = 0;
+      int nx, ny, batch, spg_gen, spg_out, cols, rows, erow = 0, ecol = 0;
     };
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
                                        {768, 768, 56, 16, 4, 4, 2},
@@ -936,8 +962,8 @@ int holo_create(holo_handle* out, int device) {
     for (const Shipped& e : kShipped) {
       Tuning t;
       t.batch = e.batch;
-      t.spg_cols = e.spg_cols;
-      t.spg_rows = e.spg_rows;
+      t.spg_gen = e.spg_gen;
+      t.spg_out = e.spg_out;
       t.cols = e.cols;
       t.rows = e.rows;
       t.erow = e.erow;
@@ -979,14 +1005,14 @@ int holo_destroy(holo_handle h) {
   return HOLO_OK;
 }
 
-int holo_set_tuning(holo_handle h, int batch, int spg_cols, int spg_rows, int cols_per_wg,
+int holo_set_tuning(holo_handle h, int batch, int spg_gen, int spg_out, int cols_per_wg,
                     int rows_per_wg) {
   if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
   if (cols_per_wg > 0 && (cols_per_wg & (cols_per_wg - 1)) != 0)
     return fail(HOLO_ERR_ARG, "cols_per_wg must be a power of two");
   h->tune.batch = batch;
-  h->tune.spg_cols = spg_cols;
-  h->tune.spg_rows = spg_rows;
+  h->tune.spg_gen = spg_gen;
+  h->tune.spg_out = spg_out;
   h->tune.cols = cols_per_wg;
   h->tune.rows = rows_per_wg;  // cached GS graphs are keyed by their tiles and stay valid
   return HOLO_OK;
@@ -1659,15 +1685,15 @@ int holo_profile_read(holo_handle h, int pass, double* total_ms, long long* laun
   return HOLO_OK;
 }
 
-int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_cols, int spg_rows,
+int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_gen, int spg_out,
                              int cols_per_wg, int rows_per_wg) {
   if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
   if (cols_per_wg > 0 && (cols_per_wg & (cols_per_wg - 1)) != 0)
     return fail(HOLO_ERR_ARG, "cols_per_wg must be a power of two");
   Tuning t;
   t.batch = batch;
-  t.spg_cols = spg_cols;
-  t.spg_rows = spg_rows;
+  t.spg_gen = spg_gen;
+  t.spg_out = spg_out;
   t.cols = cols_per_wg;
   t.rows = rows_per_wg;
   h->size_tune[{nx, ny}] = t;
@@ -1681,8 +1707,8 @@ int holo_get_tuning_for_size(holo_handle h, int nx, int ny, int slices, int* out
   HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 512));
   const ReconPlan rp = recon_plan(h, g, slices > 0 ? slices : 1 << 20);
   out5[0] = rp.m;
-  out5[1] = rp.spg_c;
-  out5[2] = rp.spg_r;
+  out5[1] = rp.spg_gen;
+  out5[2] = rp.spg_out;
   out5[3] = g.W;
   out5[4] = g.rows;
   return HOLO_OK;
@@ -1836,21 +1862,23 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     if ((bi >= 1 && t.batch == tried[0]) || (bi == 2 && t.batch == tried[1])) continue;
     double ba = 1e30, bb = 1e30;
     Tuning ta = t, tb = t;
+    // pass A (columns, generates H_k): columns per workgroup and slices per workgroup
     for (int W : {2, 4, 8, 16})
-      for (int sc : {4, 8, 16}) {
+      for (int sg : {4, 8, 16}) {
         Tuning c = t;
         c.cols = W;
-        c.spg_cols = std::min(sc, t.batch);
-        c.spg_rows = 8;
+        c.spg_gen = std::min(sg, t.batch);
+        c.spg_out = 8;
         double v;
         if (time_round(c, true, false, &v) != HOLO_OK) continue;
         if (v < ba) { ba = v; ta = c; }
       }
+    // pass B (rows, forms the projection): rows per workgroup and slices per workgroup
     for (int rows : {1, 2, 4, 8})
-      for (int sr : {4, 8, 16}) {
+      for (int so : {4, 8, 16}) {
         Tuning c = ta;
         c.rows = rows;
-        c.spg_rows = std::min(sr, t.batch);
+        c.spg_out = std::min(so, t.batch);
         double v;
         if (time_round(c, false, true, &v) != HOLO_OK) continue;
         if (v < bb) { bb = v; tb = c; }

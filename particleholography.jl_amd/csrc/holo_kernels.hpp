@@ -55,13 +55,13 @@ __device__ __forceinline__ double axis_a_s(int n, int N, double s) {
   return t * t;
 }
 
-// exp(2 pi i u) for u in cycles, with u wrapped to [-1/2, 1/2] in FP64 before the FP32
-// sincospi. Evanescent components (d < 0) are returned as 0.
+// exp(2 pi i u) for u in cycles, with u wrapped to f = u - rint(u) in [-1/2, 1/2] in FP64, then
+// the hardware sine and cosine (v_sin_f32 / v_cos_f32 take their argument in revolutions; over
+// [-1/2, 1/2] measured on MI355X against FP64: max abs error 1.25e-7, RMS 3.5e-8; sincospif
+// gave 5.1e-8 at about 35 instructions per pair instead of 2).
 __device__ __forceinline__ float2 cis_cycles(double u) {
-  const double f = u - rint(u);
-  float s, c;
-  sincospif(static_cast<float>(2.0 * f), &s, &c);
-  return make_float2(c, s);
+  const float f = static_cast<float>(u - rint(u));
+  return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -231,6 +231,45 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
 }
 
+// Column-tile accesses: element (row t + T*e, column ci) of a plane at a uniform base. The row
+// term of each element folds into a scalar base pointer and the thread keeps one 32-bit offset
+// t*nx + ci, so the loads and stores use scalar-base addressing instead of one 64-bit address
+// register pair per element (16 pairs per thread at E = 16 held 60-120 extra VGPRs).
+template <int E>
+__device__ __forceinline__ void col_load(float2 (&v)[E], const float2* __restrict__ base, int nx, int T,
+                                         unsigned off) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = (base + static_cast<size_t>(e) * T * nx)[off];
+}
+template <int E>
+__device__ __forceinline__ void col_store(const float2 (&v)[E], float2* __restrict__ base, int nx, int T,
+                                          unsigned off) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) (base + static_cast<size_t>(e) * T * nx)[off] = v[e];
+}
+
+// The same accesses through a buffer resource (gfx9 raw buffer: base, byte range, no stride):
+// one 32-bit offset register per thread, the element's row term in the scalar offset, and
+// accesses past `bytes` dropped by the hardware (stores) or returning 0 (loads).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+template <int E>
+__device__ __forceinline__ void col_load_buf(float2 (&v)[E], __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                             unsigned row_bytes) {
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    v[e] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, e * row_bytes, 0));
+}
+template <int E>
+__device__ __forceinline__ void col_store_buf(const float2 (&v)[E], __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                              unsigned row_bytes) {
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[e]), r, voff, e * row_bytes, 0);
+}
+
 template <int E, int L, bool STAGE = true>
 __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   ColCtx c;
@@ -359,10 +398,16 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
+  const unsigned off = static_cast<unsigned>(c.t * g.nx + ci);
+  const unsigned pbytes = static_cast<unsigned>(static_cast<size_t>(g.nx) * g.ny * sizeof(float2));
+  const unsigned rowb = static_cast<unsigned>(T * g.nx * sizeof(float2));
   float2 f[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) f[e] = Fsrc[static_cast<size_t>(c.t + T * e) * g.nx + ci];
-  if constexpr (HMODE == 0) stage_twiddles(c.tw, g.twy, g.py.ntw);
+  if constexpr (HMODE == 0) {
+    col_load_buf(f, plane_rsrc(Fsrc, pbytes), off * 8u, rowb);
+    stage_twiddles(c.tw, g.twy, g.py.ntw);
+  } else {
+    col_load(f, Fsrc, g.nx, T, off);
+  }
 
   if constexpr (HMODE == 0) {
     // f <- F .* H_{k0} and r = exp(2 pi i dzl sqrt(d)), both from FP64-wrapped phases; then
@@ -398,11 +443,8 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
         for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
       }
       fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
-      if (ok) {
-        float2* __restrict__ dst = G + static_cast<size_t>(k) * a.gstride;
-#pragma unroll
-        for (int e = 0; e < E; ++e) dst[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
-      }
+      // columns past the plane (the last tile) get an offset past the buffer: dropped stores
+      col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? off * 8u : pbytes, rowb);
     }
   } else {
     float2 hdz[E];

@@ -154,8 +154,8 @@ class Handle:
         except Exception:
             pass
 
-    def set_tuning(self, batch=0, spg_cols=0, spg_rows=0, cols_per_wg=0, rows_per_wg=0):
-        check(self.lib.holo_set_tuning(self.ptr, batch, spg_cols, spg_rows, cols_per_wg, rows_per_wg))
+    def set_tuning(self, batch=0, spg_gen=0, spg_out=0, cols_per_wg=0, rows_per_wg=0):
+        check(self.lib.holo_set_tuning(self.ptr, batch, spg_gen, spg_out, cols_per_wg, rows_per_wg))
 
     def set_workspace_limit(self, nbytes: int):
         check(self.lib.holo_set_workspace_limit(self.ptr, int(nbytes)))

@@ -75,7 +75,7 @@ def _device_key(device: int) -> str:
     return torch.cuda.get_device_name(device)
 
 
-FIELDS = ("batch", "spg_cols", "spg_rows", "cols_per_wg", "rows_per_wg")
+FIELDS = ("batch", "spg_gen", "spg_out", "cols_per_wg", "rows_per_wg")
 
 
 def autotune(nx: int, ny: int, wavlen: float = 0.6328, dx: float = 10.0, device: int = 0,

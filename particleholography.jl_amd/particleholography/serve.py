@@ -118,7 +118,7 @@ def select_batch(waiting: deque, used: int, budget: int, max_batch: int, others_
 # they fill those tails, and 16-slice rounds serve more (capacity of the 1000-request stream on
 # MI355X, 1024^2-equivalent slices/s: four slots 171k against 159k with 64-slice rounds, two
 # slots 176k against 172k).
-SLOT_TUNING = {(2048, 2048): (16, 16, 4, 4, 2)}  # batch, spg_cols, spg_rows, cols, rows
+SLOT_TUNING = {(2048, 2048): (16, 16, 4, 4, 2)}  # batch, spg_gen, spg_out, cols, rows
 
 
 class Server:

@@ -86,7 +86,7 @@ def test_autotune_keeps_results_correct(gpu):
     choice = batch.autotune(n, n, LAM, DX, save=False)
     assert choice["batch"] >= 1 and choice["cols_per_wg"] in (1, 2, 4, 8, 16)
     eff = batch.get_tuning_for_size(n, n, 1000)
-    assert eff["batch"] == choice["batch"] and eff["spg_cols"] == choice["spg_cols"]
+    assert eff["batch"] == choice["batch"] and eff["spg_gen"] == choice["spg_gen"]
     # the reported choice is the tiles in force, defaults resolved (also when the incumbent,
     # the default round of a size without an entry, keeps its place)
     assert batch.get_tuning_for_size(n, n, 0) == choice
