@@ -283,6 +283,10 @@ int holo_set_workspace_limit(holo_handle h, size_t bytes);
  * need (spectrum, a round of column-transformed slices, GS tables) and is freed by
  * holo_destroy. No counterpart in the reference (CUDA.jl's pool owns its temporaries). */
 int holo_workspace_size(holo_handle h, size_t* bytes);
+/* Grows the handle's workspace to at least `bytes` now (no counterpart in the reference). A
+ * call that needs more than the handle holds grows it itself, which synchronizes the device;
+ * a server reserves its slots' workspaces up front so that no call does that mid-stream. */
+int holo_reserve_workspace(holo_handle h, size_t bytes);
 /* Per-size tile choices (0 = default): slices per batch, slices per column / row workgroup,
  * columns per workgroup, rows per workgroup. The global holo_set_tuning overrides them. */
 int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_gen, int spg_out,

@@ -158,6 +158,45 @@ def reconst_xyprojection(wavefront, slices, mode="exact", **kw) -> np.ndarray:
     return proj.astype(np.float32)
 
 
+def reconst_xyprojection_running(wavefront, slices, *, z_front, z_dz, wavlen, dx, workers=None) -> np.ndarray:
+    """The FP64 projection of ``reconst_xyprojection(mode='exact')`` for long slice stacks
+    (holofunc.jl:248-266): the spectrum is advanced from slice to slice by the complex128 product
+    with H(z_dz) (the composition identity H(a) H(b) = H(a + b) of propagating components; FP64
+    drift over 1000 products ~1e-13), and the FFTs run on ``workers`` threads (scipy.fft, FP64).
+    Pinned against the per-slice evaluation in tests/test_oracle.py."""
+    import scipy.fft as sfft
+
+    if workers is None:
+        try:
+            import os
+
+            workers = max(1, min(16, len(os.sched_getaffinity(0))))
+        except AttributeError:  # pragma: no cover
+            workers = 1
+    ny, nx = wavefront.shape
+    spec = sfft.fft2(wavefront.astype(np.complex128), workers=workers)
+    d = transfer_sqrt_arr(nx, wavlen, dx, ny, dtype=np.float64)
+    s = natural(np.sqrt(np.maximum(d, 0.0)))
+    evan = natural(d < 0)
+
+    def cis(z):
+        theta = (TWO_PI * z / wavlen) * s
+        h = np.cos(theta) + 1j * np.sin(theta)
+        h[evan] = 0.0
+        return h
+
+    g = spec * cis(z_front)
+    hdz = cis(z_dz)
+    proj = None
+    for k in range(slices):
+        if k:
+            g *= hdz
+        psi = sfft.ifft2(g, workers=workers)
+        inten = psi.real ** 2 + psi.imag ** 2
+        proj = inten if proj is None else np.fmin(proj, inten, out=proj)
+    return proj.astype(np.float32)
+
+
 def n0f8(v: np.ndarray) -> np.ndarray:
     """reinterpret(N0f8, round(UInt8, clamp(v,0,1)*255)) with ties-to-even
     (holofunc.jl:154-156). ``v`` is promoted to FP64 first (clamp with 0.0/1.0 literals)."""

@@ -412,9 +412,11 @@ dim3 row_block(const Geom& g) { return dim3(g.rows * g.px.T); }
 dim3 col_grid(const Geom& g, int y = 1) { return dim3((g.nx + g.W - 1) / g.W, y); }
 dim3 col_block(const Geom& g) { return dim3(g.W * g.py.T); }
 
-// Grow-only workspace. Growing preserves the old contents (callers may keep data at the
-// start of the workspace across a call that needs more room).
-int ensure_ws(holo_context* h, size_t bytes) {
+// Grow-only workspace. Growing keeps the first `keep` bytes (a caller's data at the start of
+// the workspace, e.g. the padded wavefront, across a call that needs more room). The old buffer
+// may still be read by work enqueued on any stream the handle served, so growth waits for the
+// device before freeing it; holo_reserve_workspace lets a caller grow it before such work.
+int ensure_ws(holo_context* h, size_t bytes, size_t keep = 0) {
   if (bytes <= h->ws_bytes) return HOLO_OK;
   void* nw = nullptr;
   HOLO_HIP(hipDeviceSynchronize());
@@ -423,7 +425,7 @@ int ensure_ws(holo_context* h, size_t bytes) {
     return fail(HOLO_ERR_NOMEM, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
   }
   if (h->ws) {
-    HOLO_HIP(hipMemcpy(nw, h->ws, h->ws_bytes, hipMemcpyDeviceToDevice));
+    if (keep > 0) HOLO_HIP(hipMemcpy(nw, h->ws, std::min(keep, h->ws_bytes), hipMemcpyDeviceToDevice));
     HOLO_HIP(hipFree(h->ws));
   }
   h->ws = nw;
@@ -582,7 +584,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     ws = fixed->buf;
   } else {
     const size_t need = reserve + plane * sizeof(float2) * (1 + static_cast<size_t>(rp.m) + (arrays ? 1 : 0));
-    HOLO_TRY(ensure_ws(h, need));
+    HOLO_TRY(ensure_ws(h, need, reserve));
     ws = h->ws;
   }
   float2* F = reinterpret_cast<float2*>(static_cast<char*>(ws) + reserve);
@@ -600,8 +602,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     for (int b0 = 0; b0 < slices; b0 += rp.m) {
       JobTable jt{};
       jt.n = 1;
-      jt.d[0] = JobDesc{F, G, proj, (front->z + static_cast<double>(b0) * dz->z) / front->wavlen,
-                        dz->z / front->wavlen, std::min(rp.m, slices - b0)};
+      jt.d[0] = JobDesc{F, G, proj, front->z / front->wavlen, dz->z / front->wavlen, std::min(rp.m, slices - b0), b0};
       HOLO_TRY(launch_round(h, g, Ex, Ey, rp, front->wavlen, front->dx, jt, jt.d[0].nbatch, st, true, true,
                             fixed == nullptr));
     }
@@ -618,12 +619,7 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     ra.gstride = plane;
     ra.nbatch = nb;
     ra.spg = rp.spg_gen;
-    if (!arrays) {
-      ra.gp.wavlen = front->wavlen;
-      ra.gp.dx = front->dx;
-      ra.gp.zl0 = (front->z + static_cast<double>(b0) * dz->z) / front->wavlen;
-      ra.gp.dzl = dz->z / front->wavlen;
-    } else {
+    if (arrays) {
       ra.H0 = static_cast<const float2*>(front->data);
       ra.Hdz = static_cast<const float2*>(dz->data);
       ra.run = run;
@@ -635,9 +631,10 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     jt.d[0].F = F;
     jt.d[0].G = G;
     jt.d[0].proj = proj;
-    jt.d[0].zl0 = ra.gp.zl0;
-    jt.d[0].dzl = ra.gp.dzl;
+    jt.d[0].zl0 = arrays ? 0.0 : front->z / front->wavlen;
+    jt.d[0].dzl = arrays ? 0.0 : dz->z / front->wavlen;
     jt.d[0].nbatch = nb;
+    jt.d[0].kbase = b0;
     if (!arrays) {
       HOLO_TRY(launch_gen_cols(h, g, Ey, rp, front->wavlen, front->dx, jt, nb, st));
     } else {
@@ -1783,7 +1780,8 @@ int holo_reconst_batch(holo_handle h, void* stream, const holo_job* jobs, int nj
           d.F = F[q - c0];
           d.G = F[q - c0] + plane;
           d.proj = j.proj;
-          d.zl0 = (j.front.z + static_cast<double>(b0) * j.dz.z) / j.front.wavlen;
+          d.zl0 = j.front.z / j.front.wavlen;
+          d.kbase = b0;
           d.dzl = j.dz.z / j.front.wavlen;
           d.nbatch = std::min(rp.m, j.slices - b0);
           maxnb = std::max(maxnb, d.nbatch);
@@ -1823,12 +1821,12 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     // batch 0 (the default round, e.g. an incumbent without a size entry) is timed at the
     // round size a long reconstruction would use, not capped at zero slices
     const ReconPlan rp = recon_plan(h, g, t.batch > 0 ? t.batch : 1 << 20);
-    HOLO_TRY(ensure_ws(h, pbytes * (1 + static_cast<size_t>(rp.m)) + plane * sizeof(float)));
+    HOLO_TRY(ensure_ws(h, pbytes * (1 + static_cast<size_t>(rp.m)) + plane * sizeof(float), pbytes));
     float2* F = static_cast<float2*>(h->ws);
     float* proj = reinterpret_cast<float*>(F + plane * (1 + rp.m));
     JobTable jt{};
     jt.n = 1;
-    jt.d[0] = JobDesc{F, F + plane, proj, -1e4 / wavlen, -200.0 / wavlen, rp.m};
+    jt.d[0] = JobDesc{F, F + plane, proj, -1e4 / wavlen, -200.0 / wavlen, rp.m, 0};
     HOLO_TRY(launch_round(h, g, Ex, Ey, rp, wavlen, dx, jt, rp.m, st, a, b));  // warm-up
     HOLO_HIP(hipEventRecord(e0, st));
     constexpr int kReps = 3;
@@ -1918,6 +1916,11 @@ int holo_workspace_size(holo_handle h, size_t* bytes) {
   if (!h || !bytes) return fail(HOLO_ERR_ARG, "NULL argument");
   *bytes = h->ws_bytes;
   return HOLO_OK;
+}
+
+int holo_reserve_workspace(holo_handle h, size_t bytes) {
+  if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
+  return ensure_ws(h, bytes);
 }
 
 int holo_set_workspace_limit(holo_handle h, size_t bytes) {

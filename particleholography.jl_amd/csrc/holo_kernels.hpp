@@ -33,8 +33,7 @@ struct Geom {
 //   d = (1 - a_x) - a_y;  H = exp(2 pi i frac(z/wavlen * sqrt(d)))
 struct GenParams {
   double wavlen, dx;
-  double zl0;   // z / wavlen of the first slice handled by the launch (cycles per unit sqrt)
-  double dzl;   // dz / wavlen between slices
+  double zl0;   // z / wavlen of a single-plane propagation (cycles per unit sqrt(d))
 };
 
 __device__ __forceinline__ double axis_a(int n, int N, double wavlen, double dx) {
@@ -343,7 +342,8 @@ __global__ __launch_bounds__(512) void k_cols_fft(const float2* __restrict__ in,
 // ------------------------------------------------------------------------------------------
 // Reconstruction pass A (column inverse FFT of F .* H_k), A6-A9
 // ------------------------------------------------------------------------------------------
-// HMODE 0 (generated): H_k = exp(2 pi i frac((zl0 + k*dzl) * sqrt(d))). A workgroup owns
+// HMODE 0 (generated): H_k = exp(2 pi i frac((zl0 + k*dzl) * sqrt(d))) for the hologram's slice k.
+//   A workgroup owns
 //   `spg` consecutive slices of a W x ny tile: it evaluates H at its first slice and the
 //   per-slice rotation exp(2 pi i frac(dzl * sqrt(d))) from FP64-wrapped phases, then advances
 //   F .* H_k by one ComplexF32 multiply per slice (relative drift <= spg * 1.2e-7).
@@ -372,9 +372,10 @@ struct JobDesc {
   const float2* F;  // natural-order spectrum
   float2* G;        // this hologram's batch of column-transformed slices
   float* proj;      // its projection
-  double zl0;       // z / wavlen of its first slice in this round
+  double zl0;       // z / wavlen of the hologram's slice 0
   double dzl;       // dz / wavlen
   int nbatch;       // its slices in this round
+  int kbase;        // index of the round's first slice in the hologram's stack
 };
 struct JobTable {
   int n;
@@ -414,7 +415,9 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
     // every further slice of the group advances f <- f .* r (one complex multiply).
     const int k0 = blockIdx.y * a.spg;
     const int k1 = min(k0 + a.spg, jd.nbatch);
-    const double zl = jd.zl0 + k0 * jd.dzl;
+    // the group's phase origin depends only on its absolute slice index, so round sizes that
+    // are multiples of `spg` give bit-identical slices
+    const double zl = jd.zl0 + static_cast<double>(jd.kbase + k0) * jd.dzl;
     const double dzl = jd.dzl;
     float2* __restrict__ G = jd.G;
     float2 r[E];

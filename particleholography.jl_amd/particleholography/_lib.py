@@ -86,6 +86,7 @@ SIGNATURES = {
     "holo_phase_retrieval_batch": [_vp, _vp, ctypes.POINTER(holo_gs_job), _i],
     "holo_set_tuning_for_size": [_vp, _i, _i, _i, _i, _i, _i, _i],
     "holo_set_workspace_limit": [_vp, ctypes.c_size_t],
+    "holo_reserve_workspace": [_vp, ctypes.c_size_t],
     "holo_workspace_size": [_vp, ctypes.POINTER(ctypes.c_size_t)],
     "holo_get_tuning_for_size": [_vp, _i, _i, _i, ctypes.POINTER(ctypes.c_int)],
     "holo_autotune": [_vp, _vp, _i, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
@@ -159,6 +160,10 @@ class Handle:
 
     def set_workspace_limit(self, nbytes: int):
         check(self.lib.holo_set_workspace_limit(self.ptr, int(nbytes)))
+
+    def reserve_workspace(self, nbytes: int):
+        """Allocates the handle's device workspace up front (at least ``nbytes``)."""
+        check(self.lib.holo_reserve_workspace(self.ptr, int(nbytes)))
 
     def workspace_size(self) -> int:
         out = ctypes.c_size_t()

@@ -153,6 +153,11 @@ def phase_retrieval_batch(jobs: List[GSJob], device: int = 0, handle=None) -> Li
         arr[i].nx, arr[i].ny = nx, ny
         arr[i].holo1, arr[i].holo2 = h1.data_ptr(), h2.data_ptr()
         if j.fwd is not None:
+            for name, a in (("fwd", j.fwd), ("bwd", j.bwd)):
+                if not isinstance(a, torch.Tensor) or not a.is_cuda or a.device != h1.device:
+                    raise AssertionError(f"job {i}: transfer {name} must be a tensor on {h1.device}")
+                if a.dtype != torch.complex64 or tuple(a.shape) != (ny, nx):
+                    raise AssertionError(f"job {i}: transfer {name} must be complex64 of shape {(ny, nx)}")
             f, b = j.fwd.contiguous(), j.bwd.contiguous()
             keep += [f, b]
             arr[i].fwd = holo_tf(f.data_ptr(), 0.0, 0.0, 0.0, 0)

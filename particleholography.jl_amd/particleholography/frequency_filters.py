@@ -40,10 +40,13 @@ def cu_super_gaussian_filter(prop_dist: float, wavlen: float, imglen: int, pixel
 
 
 def _apply(src: torch.Tensor, lpf: CuLowPassFilter, dst: torch.Tensor) -> None:
+    """The filter must be a Float32 tensor of the wavefront's shape on its device (the reference's
+    CuLowPassFilter{Float32}, types.jl:50-52); a non-contiguous one is copied."""
+    device = _dev(src)
+    if not isinstance(lpf.data, torch.Tensor) or _dev(lpf.data) != device:
+        raise AssertionError(f"low-pass filter must be a tensor on cuda:{device}")
     filt = _contig(lpf.data, torch.float32)
     assert tuple(filt.shape) == tuple(src.shape), "low-pass filter and wavefront must have the same size"
-    device = _dev(src)
-    _dev(filt)
     ny, nx = src.shape
     check(_lib.load().holo_apply_low_pass_filter(_h(device), _stream(device), nx, ny, _p(dst),
                                                  _p(src), _p(filt)))

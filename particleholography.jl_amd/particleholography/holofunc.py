@@ -119,12 +119,29 @@ def _contig(t: torch.Tensor, dtype) -> torch.Tensor:
     return t.contiguous()
 
 
-def _tf_struct(tf: CuTransfer, use_generated: bool) -> holo_tf:
+def _tf_struct(tf: CuTransfer, use_generated: bool, shape, device: int) -> holo_tf:
+    """The C ABI's holo_tf for a transfer. An array-backed transfer (read by the kernels) must be a
+    ComplexF32 (ny, nx) tensor on the call's device, as the reference's CuTransfer{ComplexF32}
+    (types.jl:25-27); a non-contiguous one is copied, and the copy stays referenced by the
+    returned struct until the call has been enqueued (the caching allocator keeps it valid for
+    the stream's later work)."""
     s = holo_tf()
     if use_generated:
         s.data, s.z, s.wavlen, s.dx, s.generated = None, float(tf.z), float(tf.wavlen), float(tf.dx), 1
-    else:
-        s.data, s.z, s.wavlen, s.dx, s.generated = tf.data.data_ptr(), 0.0, 0.0, 0.0, 0
+        return s
+    a = tf.data
+    if not isinstance(a, torch.Tensor) or a.dtype != torch.complex64:
+        raise AssertionError(f"transfer arrays must be ComplexF32 (torch.complex64) tensors, got "
+                             f"{getattr(a, 'dtype', type(a))}")
+    if tuple(a.shape) != tuple(shape):
+        raise AssertionError(f"transfer array must be {tuple(shape)}, got {tuple(a.shape)}")
+    if not a.is_cuda:
+        raise AssertionError("transfer arrays must be device (GPU) tensors")
+    if (a.device.index or 0) != device:
+        raise AssertionError(f"transfer array on cuda:{a.device.index or 0}, call on cuda:{device}")
+    a = a.contiguous()
+    s.data, s.z, s.wavlen, s.dx, s.generated = a.data_ptr(), 0.0, 0.0, 0.0, 0
+    s._keep = a
     return s
 
 
@@ -202,7 +219,7 @@ def cu_phase_retrieval_holo(holo1: torch.Tensor, holo2: torch.Tensor, transfer: 
     holo2 = _contig(holo2, torch.float32)
     device = _dev(holo1)
     gen = _choose_generated(h_mode, transfer, invtransfer)
-    f, b = _tf_struct(transfer, gen), _tf_struct(invtransfer, gen)
+    f, b = _tf_struct(transfer, gen, (datlen, datlen), device), _tf_struct(invtransfer, gen, (datlen, datlen), device)
     out = torch.empty((datlen, datlen), dtype=torch.complex64, device=holo1.device)
     check(_lib.load().holo_phase_retrieval(_h(device), _stream(device), datlen, datlen, _p(holo1), _p(holo2),
                                            ctypes.byref(f), ctypes.byref(b), int(priter), _p(out),
@@ -240,7 +257,7 @@ def cu_get_reconst_xyprojection(wavefront: CuWavefront, transfer_front: CuTransf
     device = _dev(wf)
     ny, nx = wf.shape
     gen = _choose_generated(h_mode, transfer_front, transfer_dz)
-    f, d = _tf_struct(transfer_front, gen), _tf_struct(transfer_dz, gen)
+    f, d = _tf_struct(transfer_front, gen, (ny, nx), device), _tf_struct(transfer_dz, gen, (ny, nx), device)
     proj = torch.empty((ny, nx), dtype=torch.float32, device=wf.device)
     check(_lib.load().holo_reconst_xyprojection(_h(device), _stream(device), nx, ny, _p(wf), ctypes.byref(f),
                                                 ctypes.byref(d), int(slices), _p(proj)))
@@ -255,7 +272,7 @@ def reconst_xyprojection_holo(holo: torch.Tensor, transfer_front: CuTransfer, tr
     ny, nx = holo.shape
     assert (ny, nx) == tuple(transfer_front.data.shape) == tuple(transfer_dz.data.shape), "size mismatch"
     gen = _choose_generated(h_mode, transfer_front, transfer_dz)
-    f, d = _tf_struct(transfer_front, gen), _tf_struct(transfer_dz, gen)
+    f, d = _tf_struct(transfer_front, gen, (ny, nx), device), _tf_struct(transfer_dz, gen, (ny, nx), device)
     proj = out if out is not None else torch.empty((ny, nx), dtype=torch.float32, device=holo.device)
     check(_lib.load().holo_reconst_xyprojection_holo(_h(device), _stream(device), nx, ny, _p(holo),
                                                      ctypes.byref(f), ctypes.byref(d), int(slices), _p(proj)))
@@ -274,7 +291,7 @@ def _reconst_vol(wavefront, transfer_front, transfer_dz, slices, return_type, wi
     device = _dev(wf)
     ny, nx = wf.shape
     gen = _choose_generated(h_mode, transfer_front, transfer_dz)
-    f, d = _tf_struct(transfer_front, gen), _tf_struct(transfer_dz, gen)
+    f, d = _tf_struct(transfer_front, gen, (ny, nx), device), _tf_struct(transfer_dz, gen, (ny, nx), device)
     vol = torch.empty((int(slices), ny, nx), dtype=tdtype, device=wf.device)
     proj = torch.empty((ny, nx), dtype=torch.float32, device=wf.device) if with_proj else None
     check(_lib.load().holo_reconst_vol(_h(device), _stream(device), nx, ny, _p(wf), ctypes.byref(f),
@@ -315,7 +332,8 @@ def cu_get_reconst_vol_and_xyprojection_padded(wavefront: CuWavefront, transfer_
     device = _dev(wf)
     ny, nx = wf.shape
     gen = _choose_generated(h_mode, transfer_front, transfer_dz)
-    f, d = _tf_struct(transfer_front, gen), _tf_struct(transfer_dz, gen)
+    f, d = (_tf_struct(transfer_front, gen, (2 * ny, 2 * nx), device),
+            _tf_struct(transfer_dz, gen, (2 * ny, 2 * nx), device))
     vol = torch.empty((int(slices), ny, nx), dtype=torch.uint8, device=wf.device)
     proj = torch.empty((ny, nx), dtype=torch.float32, device=wf.device)
     check(_lib.load().holo_reconst_vol_and_xyprojection_padded(_h(device), _stream(device), nx, ny, _p(wf),
@@ -332,10 +350,10 @@ def cu_asm_prop_(outholo: CuWavefront, inholo: CuWavefront, d_sqr: CuTransferSqr
     device = _dev(src)
     if d_sqr.dx is not None and d_sqr.wavlen == float(wavlen):
         tf = CuTransfer(d_sqr.data, z=float(zprop), wavlen=float(wavlen), dx=d_sqr.dx)
-        s = _tf_struct(tf, True)
+        s = _tf_struct(tf, True, (datlen, datlen), device)
     else:
         tf = cu_transfer(zprop, datlen, wavlen, d_sqr)
-        s = _tf_struct(tf, False)
+        s = _tf_struct(tf, False, (datlen, datlen), device)
     dst = outholo.data
     tmp = dst if dst.is_contiguous() and dst.dtype == torch.complex64 else torch.empty_like(src)
     check(_lib.load().holo_asm_prop(_h(device), _stream(device), datlen, datlen, _p(tmp), _p(src), ctypes.byref(s)))

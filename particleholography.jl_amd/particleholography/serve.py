@@ -139,8 +139,14 @@ class Server:
         self.slots = [_Slot(torch.cuda.Stream(device=self.dev), _lib.Handle(device)) for _ in range(n_slots)]
         for s in self.slots:
             s.handle.set_workspace_limit(workspace_per_slot)
+            # the whole workspace up front: the batched calls plan within the limit, so no call
+            # grows it (growth synchronizes the device and would drain the other slots' work)
+            s.handle.reserve_workspace(workspace_per_slot)
             for (nx, ny), t in SLOT_TUNING.items():
                 check(_lib.load().holo_set_tuning_for_size(s.handle.ptr, nx, ny, *t))
+
+    def workspace_sizes(self) -> list:
+        return [s.handle.workspace_size() for s in self.slots]
 
     def _launch(self, slot: _Slot, reqs: List[Request], now: float) -> _Group:
         with torch.cuda.stream(slot.stream):
@@ -214,10 +220,16 @@ def summarize(requests: List[Request], makespan: float, groups: int = 0) -> dict
     slices = sum(r.slices for r in requests)
     eq1024 = sum(r.slices * r.holo.shape[0] * r.holo.shape[1] for r in requests) / (1024 * 1024)
     work = sum(r.work_bytes() for r in requests)
+    per_size = {}
+    for (ny, nx) in sorted({tuple(r.holo.shape) for r in requests}):
+        ls = np.array([r.t_done - r.arrival for r in requests if tuple(r.holo.shape) == (ny, nx)])
+        per_size[f"{nx}x{ny}"] = {"requests": int(ls.size), "p50_ms": float(np.percentile(ls, 50) * 1e3),
+                                  "p99_ms": float(np.percentile(ls, 99) * 1e3)}
     return {"requests": len(requests), "groups": groups, "makespan_s": makespan,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "slices_per_s": slices / makespan,
-            "slices_1024eq_per_s": eq1024 / makespan, "algorithmic_GBs": work / makespan / 1e9}
+            "slices_1024eq_per_s": eq1024 / makespan, "algorithmic_GBs": work / makespan / 1e9,
+            "per_size": per_size}
 
 
 def plain_one_at_a_time(requests: List[Request], device: int = 0) -> dict:
@@ -274,6 +286,7 @@ def summarize_closed(requests: List[Request], makespan: float) -> dict:
 
 
 SIZES = [(512, 512), (768, 768), (1024, 1024), (1280, 1024), (2048, 2048)]
+GS_Z = 2000.0  # um between the two planes of a GS request
 
 
 def make_stream(n: int, rate: float, seed: int = 105, pool: int = 3, gs_fraction: float = 0.2,
@@ -282,7 +295,8 @@ def make_stream(n: int, rate: float, seed: int = 105, pool: int = 3, gs_fraction
     """A seeded Poisson stream of ``n`` requests at ``rate`` requests/s. Holograms come from a
     pool of ``pool`` seeded scenes per size (formed on the device, copied to pinned host
     memory, as a camera would deliver them); each request draws its own size, slice count
-    (uniform in ``slices``), depth range and, for GS requests, 50 iterations at 2000 um."""
+    (uniform in ``slices``), depth range and, for GS requests, 50 iterations on the scene's two
+    recordings GS_Z apart."""
     from . import synth
 
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -290,8 +304,8 @@ def make_stream(n: int, rate: float, seed: int = 105, pool: int = 3, gs_fraction
     for si, (nx, ny) in enumerate(sizes):
         for k in range(pool):
             s = seed * 1000 + si * 10 + k
-            h1 = synth.hologram_device(s, 300, nx, ny, zmin=1e4, zmax=1e5, device=device)
-            h2 = synth.hologram_device(s + 500, 300, nx, ny, zmin=1.2e4, zmax=1.02e5, device=device)
+            # one scene recorded at the sensor and GS_Z further away: a genuine two-plane pair
+            h1, h2 = synth.gs_pair_device(s, 300, nx, ny, separation=GS_Z, zmin=1e4, zmax=1e5, device=device)
             scenes[(si, k)] = tuple(t.cpu().pin_memory() if pin else t.cpu() for t in (h1, h2))
     t, reqs = 0.0, []
     for i in range(n):
@@ -302,5 +316,5 @@ def make_stream(n: int, rate: float, seed: int = 105, pool: int = 3, gs_fraction
         h1, h2 = scenes[(si, k)]
         gs = rng.random() < gs_fraction
         reqs.append(Request(i, t, h1, -1e4, -(2e5 - 1e4) / s, s, holo2=h2 if gs else None,
-                            gs_z=2000.0 if gs else 0.0, gs_iters=gs_iters if gs else 0))
+                            gs_z=GS_Z if gs else 0.0, gs_iters=gs_iters if gs else 0))
     return reqs

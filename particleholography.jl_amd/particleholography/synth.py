@@ -134,6 +134,50 @@ def gs_pair(seed: int, ndisks: int, n: int, separation: float, zmin: float = 1e4
     return _record(psi1, rng, noise), _record(psi2, rng, noise)
 
 
+def _sensor_field_device(seed, ndisks, nx, ny, zmin, zmax, wavlen, dx, max_planes, device):
+    """The sensor-plane field of ``disk_scene(seed, ...)`` formed on the GPU (complex64 tensor):
+    disk masks applied on the device, the field carried between depth planes by libholo's
+    angular-spectrum propagation (complex FP32, FP64-wrapped transfer phase). Returns
+    (field, propagate) where propagate(field, dist) propagates in place."""
+    import ctypes
+
+    import torch
+
+    from . import _lib
+
+    x, y, z, diam = disk_scene(seed, ndisks, nx, ny, zmin, zmax, dx=dx)
+    dev = torch.device(f"cuda:{device}")
+    psi = torch.ones((ny, nx), dtype=torch.complex64, device=dev)
+    lib, h = _lib.load(), _lib.handle(device)
+    st = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+    def prop(field, dist):
+        t = _lib.holo_tf(None, float(dist), float(wavlen), float(dx), 1)
+        _lib.check(lib.holo_asm_prop(h.ptr, st, nx, ny, ctypes.c_void_p(field.data_ptr()),
+                                     ctypes.c_void_p(field.data_ptr()), ctypes.byref(t)))
+
+    zcur = None
+    for zp, members in _planes(z, zmin, zmax, max_planes):
+        if zcur is not None:
+            prop(psi, zcur - zp)
+        mask = np.ones((ny, nx), dtype=np.float32)
+        for i in members:
+            _apply_disk(mask, x[i], y[i], diam[i] / 2.0, dx)
+        psi *= torch.from_numpy(mask).to(dev)
+        zcur = zp
+    if zcur is not None:
+        prop(psi, zcur)
+    return psi, prop
+
+
+def _record_device(psi, gen, noise):
+    import torch
+
+    inten = psi.real ** 2 + psi.imag ** 2
+    inten = inten + noise * torch.randn(tuple(psi.shape), generator=gen, device=psi.device, dtype=torch.float32)
+    return inten.clamp_min(0.0).contiguous()
+
+
 def hologram_device(seed: int, ndisks: int, nx: int, ny: int | None = None, zmin: float = 1e4,
                     zmax: float = 1e5, wavlen: float = 0.6328, dx: float = 10.0, noise: float = 0.01,
                     max_planes: int = 16, device: int = 0):
@@ -141,37 +185,27 @@ def hologram_device(seed: int, ndisks: int, nx: int, ny: int | None = None, zmin
     device and the field is carried between depth planes by libholo's angular-spectrum
     propagation in complex FP32 (FP64-wrapped transfer phase). For benchmark inputs in bulk;
     the tests' parity cases use the FP64 ``hologram``. Returns a Float32 (ny, nx) device tensor."""
-    import ctypes
-
     import torch
 
-    from . import _lib
+    ny = nx if ny is None else ny
+    psi, _ = _sensor_field_device(seed, ndisks, nx, ny, zmin, zmax, wavlen, dx, max_planes, device)
+    gen = torch.Generator(device=psi.device)
+    gen.manual_seed(int(seed) + 0x9E3779B9)
+    return _record_device(psi, gen, noise)
+
+
+def gs_pair_device(seed: int, ndisks: int, nx: int, ny: int | None = None, separation: float = 2000.0,
+                   zmin: float = 1e4, zmax: float = 1e5, wavlen: float = 0.6328, dx: float = 10.0,
+                   noise: float = 0.01, max_planes: int = 16, device: int = 0):
+    """``gs_pair`` formed on the GPU: two recordings of one scene, I1 at the sensor and I2
+    ``separation`` um further from the object (I1 is the plane nearer the object,
+    holofunc.jl:90-91). Returns two Float32 (ny, nx) device tensors."""
+    import torch
 
     ny = nx if ny is None else ny
-    x, y, z, diam = disk_scene(seed, ndisks, nx, ny, zmin, zmax, dx=dx)
-    dev = torch.device(f"cuda:{device}")
-    psi = torch.ones((ny, nx), dtype=torch.complex64, device=dev)
-    lib, h = _lib.load(), _lib.handle(device)
-    st = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
-
-    def prop(dist):
-        t = _lib.holo_tf(None, float(dist), float(wavlen), float(dx), 1)
-        _lib.check(lib.holo_asm_prop(h.ptr, st, nx, ny, ctypes.c_void_p(psi.data_ptr()),
-                                     ctypes.c_void_p(psi.data_ptr()), ctypes.byref(t)))
-
-    zcur = None
-    for zp, members in _planes(z, zmin, zmax, max_planes):
-        if zcur is not None:
-            prop(zcur - zp)
-        mask = np.ones((ny, nx), dtype=np.float32)
-        for i in members:
-            _apply_disk(mask, x[i], y[i], diam[i] / 2.0, dx)
-        psi *= torch.from_numpy(mask).to(dev)
-        zcur = zp
-    if zcur is not None:
-        prop(zcur)
-    gen = torch.Generator(device=dev)
+    psi, prop = _sensor_field_device(seed, ndisks, nx, ny, zmin, zmax, wavlen, dx, max_planes, device)
+    gen = torch.Generator(device=psi.device)
     gen.manual_seed(int(seed) + 0x9E3779B9)
-    inten = psi.real ** 2 + psi.imag ** 2
-    inten = inten + noise * torch.randn((ny, nx), generator=gen, device=dev, dtype=torch.float32)
-    return inten.clamp_min(0.0).contiguous()
+    i1 = _record_device(psi, gen, noise)
+    prop(psi, separation)
+    return i1, _record_device(psi, gen, noise)

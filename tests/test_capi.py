@@ -100,3 +100,33 @@ def test_types_forward_julia_size():
 
     t = ph.CuTransferSqrtPart(torch.zeros((3, 5)))
     assert t.size() == (5, 3) and t.ndims() == 2 and t.eltype() == torch.float32
+
+
+def test_transfer_arrays_are_checked_before_any_launch():
+    """An array-backed transfer is read by the kernels: a wrong element type, shape or a host
+    tensor raises on the host (the reference's CuTransfer holds CuArray{ComplexF32,2} only)."""
+    import particleholography as ph
+    from particleholography.holofunc import _tf_struct
+
+    good = torch.zeros((8, 8), dtype=torch.complex64)
+    with pytest.raises(AssertionError, match="ComplexF32"):
+        _tf_struct(ph.CuTransfer(torch.zeros((8, 8), dtype=torch.complex128)), False, (8, 8), 0)
+    with pytest.raises(AssertionError, match="ComplexF32"):
+        _tf_struct(ph.CuTransfer(torch.zeros((8, 8), dtype=torch.float32)), False, (8, 8), 0)
+    with pytest.raises(AssertionError, match=r"\(8, 8\)"):
+        _tf_struct(ph.CuTransfer(torch.zeros((8, 4), dtype=torch.complex64)), False, (8, 8), 0)
+    with pytest.raises(AssertionError, match=r"got \(4, 8\)"):  # a transposed view of the wrong shape
+        _tf_struct(ph.CuTransfer(good[:, :4].t()), False, (8, 4), 0)
+    with pytest.raises(AssertionError, match="device"):
+        _tf_struct(ph.CuTransfer(good), False, (8, 8), 0)
+    # generated transfers carry no array
+    s = _tf_struct(ph.CuTransfer(good, z=-1e4, wavlen=0.6328, dx=10.0), True, (8, 8), 0)
+    assert s.generated == 1 and not s.data
+
+
+def test_low_pass_filter_is_checked_before_any_launch():
+    import particleholography as ph
+
+    wf = ph.CuWavefront(torch.zeros((8, 8), dtype=torch.complex64))
+    with pytest.raises(AssertionError, match="device"):
+        ph.cu_apply_low_pass_filter(wf, ph.CuLowPassFilter(torch.zeros((8, 8))))

@@ -183,6 +183,25 @@ def test_projection_c1_config_vs_fp64_oracle(gpu):
     assert err < 1e-4
 
 
+def test_projection_c2_full_workload_vs_fp64_oracle(gpu):
+    """The headline workload end to end: the seed-102 1000-disk 1024^2 hologram, 1000 slices at
+    dz = 200 um from z0 = 1e4 um (to 2.098e5 um, holofunc.jl:259-263), through the fused path
+    (sqrt fused into the forward FFT, the shipped tiles), against the FP64 oracle."""
+    ph = _ph()
+    from particleholography import synth
+
+    n, slices = 1024, 1000
+    holo = synth.hologram(102, 1000, n, zmin=1e4, zmax=2e5, wavlen=LAM, dx=DX)
+    dsq = ph.cu_transfer_sqrt_arr(n, LAM, DX)
+    h0, hdz = ph.cu_transfer(-1e4, n, LAM, dsq), ph.cu_transfer(-200.0, n, LAM, dsq)
+    got = ph.reconst_xyprojection_holo(_t(holo, gpu), h0, hdz, slices).cpu().numpy()
+    want = asm.reconst_xyprojection_running(asm.gabor_wavefront(holo), slices, z_front=-1e4, z_dz=-200.0,
+                                            wavlen=LAM, dx=DX)
+    err = asm.rel_l2(got, want)
+    print(f"C2 1024^2x1000 projection rel L2 vs FP64 oracle: {err:.3e} (max abs {np.abs(got - want).max():.3e})")
+    assert err < 1e-4
+
+
 def test_gs_c3_config_vs_fp64_oracle(gpu):
     """The GS configuration: the seeded 500-disk scene recorded at planes 2000 um apart
     (1024^2), 100 iterations on the production path (graphs, line shapes), against the FP64
@@ -312,8 +331,10 @@ def test_2d_pad_known_answer(gpu):
     np.testing.assert_array_equal(p, asm.pad2(arr.cpu().numpy()))
 
 
-@pytest.mark.parametrize("n,slices", [(4, 2), (32, 5)])
+@pytest.mark.parametrize("n,slices", [(4, 2), (32, 5), (512, 4), (1024, 3)])
 def test_padded_reconstruction(gpu, n, slices):
+    """Padded volume + projection against the FP64 oracle; 512 and 1024 run the 1024- and
+    2048-point compile-time plans with the cropped store window."""
     ph = _ph()
     h = _holo(n, 41, amp=0.5) * 0.6
     dsq = ph.cu_transfer_sqrt_arr(2 * n, LAM, DX)
@@ -324,6 +345,27 @@ def test_padded_reconstruction(gpu, n, slices):
     want_vol, _ = asm.reconst_vol_and_xyprojection_padded(asm.gabor_wavefront(h.astype(np.float32)), slices,
                                                           "exact", z_front=-100.0, z_dz=-10.0, wavlen=LAM, dx=DX)
     assert np.abs(vol.cpu().numpy().astype(int) - want_vol.astype(int)).max() <= 1
+    np.testing.assert_array_equal(proj.cpu().numpy(), asm.xy_projection_from_vol(vol.cpu().numpy()))
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+def test_padded_crop_at_zero_distance(gpu, n):
+    """Closed form of the padded reconstruction's window: at z = 0 (H = 1 for every slice) each
+    stored slice is n0f8(clamp(|psi|^2, 0, 1)) of the unpadded wavefront itself, the centre window
+    of the 2n x 2n round trip pad -> FFT -> IFFT (holofunc.jl:339-364); within one N0f8 step
+    (the round trip rounds in FP32), and exact for nearly every pixel."""
+    ph = _ph()
+    rng = np.random.default_rng(n + 3)
+    h = (0.15 + 0.7 * rng.random((n, n))).astype(np.float32)  # |psi|^2 = h in (0, 1): no clamping
+    dsq = ph.cu_transfer_sqrt_arr(2 * n, LAM, DX)
+    h0 = ph.cu_transfer(0.0, 2 * n, LAM, dsq)
+    vol, proj = ph.cu_get_reconst_vol_and_xyprojection_padded(ph.cu_gabor_wavefront(_t(h, gpu)), h0, h0, 2)
+    want = asm.n0f8(np.abs(asm.gabor_wavefront(h).astype(np.complex128)) ** 2)
+    got = vol.cpu().numpy().astype(int)
+    for k in range(2):
+        diff = np.abs(got[k] - want.astype(int))
+        assert diff.max() <= 1
+        assert (diff == 0).mean() > 0.99
     np.testing.assert_array_equal(proj.cpu().numpy(), asm.xy_projection_from_vol(vol.cpu().numpy()))
 
 

@@ -34,3 +34,60 @@ def test_tight_budget_still_serves_everything(gpu):
     stats = srv.run(reqs)
     assert stats["requests"] == 10 and stats["groups"] >= 5  # at most two requests per group fit
     assert all(r.proj is not None for r in reqs)
+
+
+def test_slot_workspaces_do_not_grow_while_serving(gpu):
+    """The slots' workspaces are reserved when the server starts: no call grows one mid-stream
+    (growth synchronizes the device, draining the other slots' work)."""
+    from particleholography import serve
+
+    reqs = serve.make_stream(12, rate=0.0, seed=9, pool=1, gs_fraction=0.25, slices=(8, 64), gs_iters=3,
+                             sizes=[(256, 256), (512, 512), (1024, 1024)])
+    ws = 512 << 20
+    srv = serve.Server(n_slots=2, max_batch=4, hbm_budget=8 << 30, workspace_per_slot=ws)
+    before = srv.workspace_sizes()
+    assert all(b >= ws for b in before)
+    srv.run(reqs)
+    assert srv.workspace_sizes() == before
+
+
+def test_c5_mix_served_equals_alone_and_fp64_oracle(gpu):
+    """The served mix at its own sizes: 512^2-2048^2 holograms, 128-1024 slices, GS requests at
+    50 iterations on genuine two-plane pairs (one scene recorded 2000 um apart). Every served
+    projection equals the same request computed alone, bit for bit; the smallest GS request and
+    the smallest reconstruction among the 512^2 and 1024^2 ones are checked against the FP64
+    oracle (projection bar 1e-4 rel L2)."""
+    from oracle import asm
+    from particleholography import serve
+
+    reqs = serve.make_stream(16, rate=0.0, seed=105, pool=1, gs_fraction=0.35)
+    srv = serve.Server(n_slots=2, max_batch=8, hbm_budget=32 << 30, workspace_per_slot=2 << 30)
+    stats = srv.run(reqs)
+    assert stats["requests"] == 16 and set(stats["per_size"]) <= {"512x512", "768x768", "1024x1024",
+                                                                    "1280x1024", "2048x2048"}
+    assert any(r.is_gs for r in reqs) and any(not r.is_gs for r in reqs)
+    assert {tuple(r.holo.shape) for r in reqs} >= {(512, 512), (1024, 1024)}
+    for r in reqs:
+        h1 = r.holo.to(gpu)
+        src = serve._single_gs(h1, r.holo2.to(gpu), r) if r.is_gs else h1
+        alone = serve._single_recon(src, r).cpu().numpy()
+        np.testing.assert_array_equal(r.proj.numpy(), alone)
+    checked = 0
+    for want_gs in (True, False):
+        sel = [r for r in reqs if r.is_gs == want_gs and tuple(r.holo.shape) in ((512, 512), (1024, 1024))]
+        if not sel:
+            continue
+        r = min(sel, key=lambda q: q.slices * q.holo.numel())
+        shape = tuple(r.holo.shape)
+        i1 = r.holo.numpy()
+        if r.is_gs:
+            psi = asm.phase_retrieval(i1, r.holo2.numpy(), r.gs_iters, z=r.gs_z, wavlen=r.wavlen, dx=r.dx)
+        else:
+            psi = asm.gabor_wavefront(i1)
+        want = asm.reconst_xyprojection_running(psi, r.slices, z_front=r.z_front, z_dz=r.z_dz, wavlen=r.wavlen,
+                                                dx=r.dx)
+        err = asm.rel_l2(r.proj.numpy(), want)
+        print(f"served {shape[1]}x{shape[0]} {'GS+' if r.is_gs else ''}{r.slices} slices vs FP64 oracle: rel L2 {err:.3e}")
+        assert err < 1e-4
+        checked += 1
+    assert checked == 2
