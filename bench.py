@@ -26,9 +26,23 @@ METRIC = "z-slices/s at 1024² CF32, % of HBM roofline; p50/p99 per-hologram lat
 HBM_PEAK_GBS = 8000.0
 LAM, DX = 0.6328, 10.0
 N, SLICES, Z0, DZ, NDISKS, SEED = 1024, 1000, 1e4, 200.0, 1000, 102
-PASS_BYTES_PER_PX = 16.0  # per slice: pass A reads the spectrum (8) and writes the column
-                          # transform (8); pass B reads it (8) and read-modify-writes the
-                          # projection (4 + 4). 32 B/px per slice for the pair.
+# Traffic models, bytes per pixel and slice:
+#   fused (this design, the roofline's algorithmic bytes): pass A writes the column-transformed
+#   slice (8) and reads the spectrum once per slice group of spg_gen slices (8 / spg_gen); pass B
+#   reads the slice (8) and read-modify-writes the projection once per group of spg_out slices
+#   (8 / spg_out). 17 B/px/slice at the default 16 / 16.
+#   reference (the per-slice sequence of holofunc.jl:259-263 as one fused pass pair would see it
+#   if the spectrum were re-read every slice): 16 + 16 = 32 B/px/slice. Reported as
+#   "reference traffic", never as a roofline.
+REFERENCE_BYTES_PER_PX = 32.0
+
+
+def pass_a_bytes(spg_gen):
+    return 8.0 + 8.0 / spg_gen
+
+
+def fused_bytes(spg_gen, spg_out):
+    return 16.0 + 8.0 / spg_gen + 8.0 / spg_out
 
 
 def log(msg):
@@ -67,6 +81,9 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3: one warm-up + --steps steps, then a 256 MiB device copy "
                          "(a kernel of known traffic to scale FETCH_SIZE / WRITE_SIZE); no extras")
+    ap.add_argument("--profile-gs-batch", action="store_true",
+                    help="for rocprofv3: batched GS at 1024^2, 32 pairs x 20 iterations per call (one warm-up "
+                         "call, then --steps calls); no other work")
     ap.add_argument("--profile-detect", action="store_true",
                     help="for rocprofv3: the particle-detection extra (1024^2 x 1000 N0f8 volume, "
                          "labels and rectangles of every slice, --steps calls); no other work")
@@ -111,6 +128,19 @@ def main():
                           "algorithmic_GBs": r["algorithmic_GBs"]}), flush=True)
         return
 
+    if args.profile_gs_batch:
+        from particleholography.batch import GSJob, phase_retrieval_batch
+
+        pairs = [synth.gs_pair_device(103000 + k, 500, N, N, separation=2000.0) for k in range(4)]
+        jobs = [GSJob(pairs[k % 4][0], pairs[k % 4][1], 2000.0, 20, LAM, DX) for k in range(32)]
+        for _ in range(1 + args.steps):
+            outs = phase_retrieval_batch(jobs)
+        torch.cuda.synchronize()
+        log("bench: batched GS profile run done")
+        print(json.dumps({"profile_gs_batch": True, "steps": args.steps,
+                          "finite": bool(all(torch.isfinite(o).all() for o in outs))}), flush=True)
+        return
+
     if args.profile_gs:
         i2 = torch.from_numpy(synth.gs_pair(103, 500, N, 2000.0)[1]).to(dev)
         hf, hb = ph.cu_transfer(2000.0, N, LAM, d_sqr), ph.cu_transfer(-2000.0, N, LAM, d_sqr)
@@ -152,24 +182,37 @@ def main():
         ms, n, units = hd.profile_read(pid)
         passes[name] = {"ms_total": ms, "launches": n, "slices": units}
     hd.set_profiling(False)
-    dom = max(("recon_cols", "recon_rows"), key=lambda k: passes[k]["ms_total"])
-    d = passes[dom]
-    alg_bytes = PASS_BYTES_PER_PX * N * N * d["slices"]
-    achieved = alg_bytes / (d["ms_total"] * 1e-3) / 1e9
-    kname = "k_recon_cols<" if dom == "recon_cols" else "k_recon_rows_proj<"
-    traffic, traffic_src = None, None
+    from particleholography import batch as _batch
+
+    tune = _batch.get_tuning_for_size(N, N, SLICES)
+    # pass A (the column pass that generates H_k, k_recon_cols) is the dominant kernel
+    d = passes["recon_cols"]
+    launches = max(d["launches"], 1)
+    avg_s = d["ms_total"] * 1e-3 / launches
+    alg_bytes = pass_a_bytes(tune["spg_gen"]) * N * N * d["slices"] / launches
+    achieved = alg_bytes / avg_s / 1e9
+    traffic, dram = None, None
     pmc_file = ROOT / "profiles" / "pmc_traffic_kernels.json"
-    if pmc_file.exists():  # HBM bytes per full launch from the committed rocprofv3 PMC runs
+    if pmc_file.exists():  # HBM bytes per launch from the committed rocprofv3 PMC runs
         pmc = json.loads(pmc_file.read_text())
         for kn, v in pmc["kernels"].items():
-            if kname in kn and f", {N}" in kn:
-                traffic, traffic_src = int(v["fetch_bytes_mean"] + v["write_bytes_mean"]), pmc_file.name
+            if "k_recon_cols<" in kn and f", {N}, 0>" in kn:
+                traffic = int(v["fetch_bytes_mean"] + v["write_bytes_mean"])
+                dram = v.get("dram_read_requests_mean")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": "k_recon_cols (pass A)" if dom == "recon_cols" else "k_recon_rows_proj (pass B)",
-                "avg_launch_ms": round(d["ms_total"] / max(d["launches"], 1), 4),
-                "bytes_per_launch": alg_bytes / max(d["launches"], 1)}
-    extras = {"passes": passes, "step_algorithmic_GBs": round(32.0 * N * N * slices_per_s / 1e9, 1)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": pmc_file.name if traffic else None,
+                "kernel": "k_recon_cols<16, 1024, 0> (pass A)", "avg_launch_ms": round(avg_s * 1e3, 4),
+                "bytes_per_launch": alg_bytes,
+                "bytes_model": f"8 B/px/slice written + the 8 B/px spectrum read once per {tune['spg_gen']}-slice group",
+                "counter_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                "dram_read_requests_per_launch": dram,
+                "reference_traffic_GBs": round(REFERENCE_BYTES_PER_PX / 2 * N * N * d["slices"] / launches / avg_s / 1e9, 1)}
+    step_bytes = fused_bytes(tune["spg_gen"], tune["spg_out"])
+    extras = {"passes": passes, "tuning": tune,
+              "step_algorithmic_GBs": round(step_bytes * N * N * slices_per_s / 1e9, 1),
+              "step_bytes_per_px_slice": step_bytes,
+              "step_reference_traffic_GBs": round(REFERENCE_BYTES_PER_PX * N * N * slices_per_s / 1e9, 1)}
 
     if not args.no_extras:
         # the plain one-slice-at-a-time GPU path: one pass-A + pass-B launch pair per slice
@@ -182,20 +225,33 @@ def main():
         extras["plain_path_slices_per_s"] = round(plain, 1)
         extras["speedup_vs_plain"] = round(slices_per_s / plain, 2)
         extras["gs_1024"] = bench_gs(ph, synth, torch, dev, d_sqr)
+        extras["gs_batched_1024"] = bench_gs_batched(torch, dev)
         extras["volumes_1024"] = bench_volumes(ph, torch, holo)
         extras["lpf_1024"] = bench_lpf(ph, torch, holo)
         extras["ingest_1024"] = bench_ingest(ph, torch, dev)
         extras["detect_reference_holograms"] = bench_detect(ph, torch, dev)
         extras["batched_64"] = bench_batched(torch, dev, args.batch_holograms)
-        extras["serving"] = bench_serving(torch, args.serve_requests)
+        extras["serving"] = bench_serving(torch, args.serve_requests, plain)
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu_baseline:
-        from oracle import cpu_baseline
+        from oracle import asm, cpu_baseline
 
         log("bench: CPU baseline (reference op sequence, torch CPU complex64)")
         cpu = cpu_baseline.measure(holo_np, -Z0, -DZ, LAM, DX, SLICES, budget_s=15.0)
         cpu["value"] = round(cpu["value"], 2)
+        # the CPU configuration: 512^2, seeded 200-disk scene, 100 slices at 1e3 um from 1e4 um
+        c1_holo = synth.hologram(101, 200, 512, zmin=1e4, zmax=1e5, wavlen=LAM, dx=DX)
+        c1 = cpu_baseline.measure(c1_holo, -1e4, -1e3, LAM, DX, 100, budget_s=6.0)
+        cpu["c1_512x512x100"] = {"value": round(c1["value"], 2), "unit": c1["unit"], "sample": c1["sample"]}
+        # checker: the measured projection of the timed workload against the FP64 oracle
+        log("bench: FP64 oracle of the 1024^2 x 1000 workload (parity of the measured projection)")
+        want = asm.reconst_xyprojection_running(asm.gabor_wavefront(holo_np), SLICES, z_front=-Z0, z_dz=-DZ,
+                                                wavlen=LAM, dx=DX)
+        got = proj.cpu().numpy()
+        parity = {"projection_rel_l2_vs_fp64_oracle": float(f"{asm.rel_l2(got, want):.3e}"),
+                  "max_abs_vs_fp64_oracle": float(f"{float(np.abs(got - want).max()):.3e}"),
+                  "bar": 1e-4, "workload": "the timed 1024^2 x 1000-slice projection"}
 
     # per-hologram latency of the headline workload: submission to completion of one 1000-slice
     # reconstruction, 20 calls each synchronised (the stream is idle at every submission)
@@ -222,7 +278,7 @@ def main():
                                "at dz=200 um from z0=1e4 um, xy min-projection", "size": [N, N],
                    "slices": SLICES, "z0_um": Z0, "dz_um": DZ, "wavelength_um": LAM, "pitch_um": DX,
                    "transfer": "generated FP64-wrapped phase", "parallelism": "single GPU"},
-        "roofline": roofline, "latency": latency, "cpu_baseline": cpu, "extras": extras,
+        "roofline": roofline, "latency": latency, "cpu_baseline": cpu, "parity": parity, "extras": extras,
     }
     print(json.dumps(out), flush=True)
 
@@ -249,15 +305,20 @@ def bench_volumes(ph, torch, holo, reps=3):
     """The volume variants on the headline hologram (1024^2, dz = 200 um from z0 = 1e4 um):
     cu_get_reconst_vol N0f8 and Float32 and cu_get_reconst_vol_and_xyprojection N0f8 over 1000
     slices, cu_get_reconst_complex_vol over 250 slices (2 GiB). Algorithmic bytes per pixel and
-    slice: pass A 16, plus the column-transformed slice read 8 and the stored value (1 / 4 / 8),
-    plus 8 for the projection's read-modify-write."""
+    slice (fused model): pass A 8 + 8/spg_gen, the column-transformed slice read 8 and the
+    stored value (1 / 4 / 8), plus 8/spg_out for the projection's read-modify-write."""
+    from particleholography import batch as _batch
+
+    t = _batch.get_tuning_for_size(N, N, SLICES)
+    a = pass_a_bytes(t["spg_gen"]) + 8.0
     d_sqr = ph.cu_transfer_sqrt_arr(N, LAM, DX)
     h0, hdz = ph.cu_transfer(-Z0, N, LAM, d_sqr), ph.cu_transfer(-DZ, N, LAM, d_sqr)
     wf = ph.cu_gabor_wavefront(holo)
-    cases = {"n0f8": (SLICES, 25, lambda s: ph.cu_get_reconst_vol(wf, h0, hdz, s)),
-             "f32": (SLICES, 28, lambda s: ph.cu_get_reconst_vol(wf, h0, hdz, s, return_type=ph.Float32)),
-             "n0f8_and_projection": (SLICES, 33, lambda s: ph.cu_get_reconst_vol_and_xyprojection(wf, h0, hdz, s)),
-             "complex": (250, 32, lambda s: ph.cu_get_reconst_complex_vol(wf, h0, hdz, s))}
+    cases = {"n0f8": (SLICES, a + 1, lambda s: ph.cu_get_reconst_vol(wf, h0, hdz, s)),
+             "f32": (SLICES, a + 4, lambda s: ph.cu_get_reconst_vol(wf, h0, hdz, s, return_type=ph.Float32)),
+             "n0f8_and_projection": (SLICES, a + 1 + 8.0 / t["spg_out"],
+                                     lambda s: ph.cu_get_reconst_vol_and_xyprojection(wf, h0, hdz, s)),
+             "complex": (250, a + 8, lambda s: ph.cu_get_reconst_complex_vol(wf, h0, hdz, s))}
     out = {}
     for name, (s, bpx, fn) in cases.items():
         _, sec = timed_steps(lambda: fn(s), reps, 1, torch)
@@ -387,10 +448,16 @@ def bench_batched(torch, dev, nholo, autotune=True, reps=1):
     dt = (time.perf_counter() - t0) / reps
     slices = sum(j.slices for j in jobs)
     px_slices = sum(j.slices * j.holo.numel() for j in jobs)
+    bpx = {}
+    for nx, ny in SIZES:
+        t = batch.get_tuning_for_size(nx, ny, 1024)
+        bpx[(nx, ny)] = fused_bytes(t["spg_gen"], t["spg_out"])
+    alg = sum(j.slices * j.holo.numel() * bpx[(j.holo.shape[1], j.holo.shape[0])] for j in jobs)
     out = {"holograms": nholo, "slices": slices, "slices_per_s": round(slices / dt, 1),
            "slices_1024eq_per_s": round(px_slices / (N * N) / dt, 1),
-           "algorithmic_GBs": round(32.0 * px_slices / dt / 1e9, 1), "wall_ms": round(dt * 1e3, 2),
-           "tuning": tuned, "per_size": {}}
+           "algorithmic_GBs": round(alg / dt / 1e9, 1), "frac_of_8TBs": round(alg / dt / 1e9 / HBM_PEAK_GBS, 3),
+           "reference_traffic_GBs": round(REFERENCE_BYTES_PER_PX * px_slices / dt / 1e9, 1),
+           "wall_ms": round(dt * 1e3, 2), "tuning": tuned, "per_size": {}}
     for nx, ny in (SIZES if reps == 1 else []):
         sub = [j for j in jobs if tuple(j.holo.shape) == (ny, nx)]
         if not sub:
@@ -402,32 +469,75 @@ def bench_batched(torch, dev, nholo, autotune=True, reps=1):
         torch.cuda.synchronize()
         d = time.perf_counter() - t0
         ss = sum(j.slices for j in sub)
-        gbs = 32.0 * nx * ny * ss / d / 1e9
+        gbs = bpx[(nx, ny)] * nx * ny * ss / d / 1e9
         out["per_size"][f"{nx}x{ny}"] = {"holograms": len(sub), "slices": ss, "slices_per_s": round(ss / d, 1),
+                                         "bytes_per_px_slice": round(bpx[(nx, ny)], 3),
                                          "algorithmic_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
     for nx, ny in SIZES:
         batch.set_tuning_for_size(nx, ny, before[(nx, ny)])
     return out
 
 
-def bench_serving(torch, nreq):
+def bench_serving(torch, nreq, plain_1024):
     """Seeded Poisson stream of mixed 512^2-2048^2 requests (128-1024 slices, 20% two-plane GS
-    with 50 iterations): capacity with every request queued at once, then the stream at 75% of
-    that capacity (p50/p99 latency), beside the plain one-at-a-time path."""
-    from particleholography import serve
+    with 50 iterations on one scene recorded 2000 um apart): capacity with every request queued
+    at once, then the stream at 75% of that capacity (p50/p99 latency, overall and per size).
+    The comparator is the plain path, as for the headline: every request alone, in arrival
+    order, its reconstruction with one pass-A + pass-B launch pair per slice (the host loop of
+    holofunc.jl:259-263) and its GS with the single-pair call; measured on the stream's first
+    150 requests (the same mix), in 1024^2-equivalent slices/s."""
+    from particleholography import _lib, serve
 
     reqs = serve.make_stream(nreq, rate=0.0, seed=105)
-    srv = serve.Server(n_slots=2, max_batch=16, hbm_budget=64 << 30, workspace_per_slot=4 << 30)
-    srv.run(reqs[: min(32, nreq)])  # warm-up: plans, workspaces
+    srv = serve.Server(n_slots=2, max_batch=16, hbm_budget=64 << 30, workspace_per_slot=4 << 30,
+                       keep_results=False)
+    srv.run(reqs[: min(32, nreq)])  # warm-up: plans, graphs
+    ws0 = srv.workspace_sizes()
     cap = srv.run(reqs)
     rate = 0.75 * cap["requests"] / cap["makespan_s"]
     reqs2 = serve.make_stream(nreq, rate=rate, seed=105)
     live = srv.run(reqs2)
-    plain = serve.plain_one_at_a_time(reqs)
+    grew = srv.workspace_sizes() != ws0
+    sample = serve.make_stream(min(150, nreq), rate=0.0, seed=105)
+    hd = _lib.handle(0)
+    hd.set_tuning(1, 1, 1, 0, 0)
+    try:
+        plain = serve.plain_one_at_a_time(sample)
+    finally:
+        hd.set_tuning()
+    eq = sum(r.slices * r.holo.shape[0] * r.holo.shape[1] for r in sample) / (N * N)
+    plain_eq = eq / plain["makespan_s"]
     r = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
     return {"capacity": r(cap), "poisson_at_75pct": r(live), "rate_req_per_s": round(rate, 2),
-            "plain_one_at_a_time": r(plain),
-            "speedup_vs_plain": round(cap["slices_per_s"] / plain["slices_per_s"], 2)}
+            "plain_per_slice": {"requests": plain["requests"], "slices_1024eq_per_s": round(plain_eq, 1),
+                                "workload": "the stream's first 150 requests, each alone, one launch pair per slice"},
+            "capacity_speedup_vs_plain": round(cap["slices_1024eq_per_s"] / plain_eq, 2),
+            "load75_speedup_vs_plain": round(live["slices_1024eq_per_s"] / plain_eq, 2),
+            "workspace_grew_after_warmup": grew}
+
+
+def bench_gs_batched(torch, dev, iters=100):
+    """Batched Gerchberg-Saxton at 1024^2: 16 and 32 pairs (four seeded two-plane scenes, 2000 um
+    apart, repeated) x 100 iterations in one call, the chunk-uniform iterations in hipGraphs.
+    Algorithmic bytes per pixel and iteration: 72 (the fused two-pass propagation with generated
+    transfers) and 88 as built (the precomputed natural-order transfer tables are read, 8 B/px
+    per column pass)."""
+    from particleholography import synth
+    from particleholography.batch import GSJob, phase_retrieval_batch
+
+    pairs = [synth.gs_pair_device(103000 + k, 500, N, N, separation=2000.0) for k in range(4)]
+    out = {}
+    for npairs in (16, 32):
+        jobs = [GSJob(pairs[k % 4][0], pairs[k % 4][1], 2000.0, iters, LAM, DX) for k in range(npairs)]
+        sec_total, _ = timed_steps(lambda: phase_retrieval_batch(jobs), 3, 1, torch)
+        it_s = npairs * iters * 3 / sec_total
+        out[f"{npairs}_pairs"] = {"pair_iterations_per_s": round(it_s, 1),
+                                  "algorithmic_GBs_72": round(72.0 * N * N * it_s / 1e9, 1),
+                                  "frac_of_8TBs_72": round(72.0 * N * N * it_s / 1e9 / HBM_PEAK_GBS, 3),
+                                  "as_built_GBs_88": round(88.0 * N * N * it_s / 1e9, 1),
+                                  "call_ms": round(sec_total / 3 * 1e3, 3)}
+    out["config"] = f"1024^2, {iters} iterations at 2000 um per pair, one batched call"
+    return out
 
 
 if __name__ == "__main__":

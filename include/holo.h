@@ -276,6 +276,11 @@ typedef struct holo_gs_job {
   void* out;         /* ComplexF32 nx*ny */
 } holo_gs_job;
 int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* jobs, int njobs);
+/* Batched GS runs the iterations that are identical for a chunk of pairs (all but each pair's
+ * last) as hipGraphs of up to 16 iterations, captured once per size, chunk layout and workspace
+ * and cached on the handle (I1 and I2 are copied into the workspace, so the graphs reference no
+ * caller memory); enable = 0 launches every pass directly. No counterpart in the reference. */
+int holo_set_gs_graphs(holo_handle h, int enable);
 /* Caps the workspace the batched calls plan for (holograms are processed in chunks that fit;
  * a single hologram larger than the cap still runs). 0 = HOLO_BATCH_WS_BYTES or 8 GiB. */
 int holo_set_workspace_limit(holo_handle h, size_t bytes);

@@ -17,6 +17,20 @@ import numpy as np
 from . import asm
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo, as lscpu prints it)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:  # pragma: no cover
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
 def cores() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -70,7 +84,7 @@ def measure(holo: np.ndarray, z0: float, dz: float, wavlen: float, dx: float, sl
         if len(times) == 1:  # the first full run sizes the repetition count
             runs = int(min(25, max(3, round(budget_s / times[0]))))
     med = float(np.median(times))
-    return {"value": sample / med, "unit": "z-slices/s", "cores": n_threads, "kind": "port",
+    return {"value": sample / med, "unit": "z-slices/s", "cores": n_threads, "kind": "port", "model": cpu_model(),
             "sample": f"{nx}x{ny} hologram, {sample} of {slices} slices, median of {runs} runs "
                       f"({sum(times):.1f} s of CPU work), torch {torch.__version__} CPU complex64 FFT "
                       f"on {n_threads} threads"}

@@ -133,13 +133,16 @@ struct holo_context {
     int nx, ny;
     const void *buf, *hb;
     int conj, W, rows, ex, ey;
-    int n;  // iterations captured
+    int n;       // iterations captured
+    int npairs;  // batched calls: pairs of the chunk (0 for the single-pair call)
+    unsigned long long conjmask;  // batched calls: pairs whose backward table is the conjugate
     bool operator<(const GsKey& o) const {
-      return std::tie(nx, ny, buf, hb, conj, W, rows, ex, ey, n) <
-             std::tie(o.nx, o.ny, o.buf, o.hb, o.conj, o.W, o.rows, o.ex, o.ey, o.n);
+      return std::tie(nx, ny, buf, hb, conj, W, rows, ex, ey, n, npairs, conjmask) <
+             std::tie(o.nx, o.ny, o.buf, o.hb, o.conj, o.W, o.rows, o.ex, o.ey, o.n, o.npairs, o.conjmask);
     }
   };
   std::map<GsKey, hipGraphExec_t> gs_graphs;
+  bool gs_batch_graphs = true;  // batched GS runs its uniform iterations as cached hipGraphs
   // per-pass HIP-event timing (holo_set_profiling)
   struct ProfRec {
     int pass;
@@ -1524,7 +1527,7 @@ int holo_reconst_vol_and_xyprojection_padded(holo_handle h, void* stream, int nx
 constexpr int kGsGraphIters = 16;
 constexpr size_t kGsGraphCache = 64;
 int gs_graph(holo_context* h, holo_context::GsKey key, int n, const Geom& g, int Ex, int Ey, GsPair* p,
-             float sc, hipGraphExec_t* out) {
+             float sc, hipGraphExec_t* out, int np = 1) {
   key.n = n;
   auto it = h->gs_graphs.find(key);
   if (it != h->gs_graphs.end()) {
@@ -1539,7 +1542,7 @@ int gs_graph(holo_context* h, holo_context::GsKey key, int n, const Geom& g, int
   hipGraph_t graph = nullptr;
   HOLO_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
   int r = HOLO_OK;
-  for (int i = 0; i < n && r == HOLO_OK; ++i) r = gs_round(h, g, Ex, Ey, p, 1, 0, sc, h->cap);
+  for (int i = 0; i < n && r == HOLO_OK; ++i) r = gs_round(h, g, Ex, Ey, p, np, 0, sc, h->cap);
   const hipError_t e = hipStreamEndCapture(h->cap, &graph);
   if (r != HOLO_OK) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -1590,7 +1593,7 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
   if (use_graph && body > 0) {
     // A graph holds up to kGsGraphIters iterations (4 launches each): graphs of one iteration
     // leave the device idle between graph launches.
-    holo_context::GsKey key{nx, ny, p.B, p.hb, p.conj_b, g.W, g.rows, Ex, Ey, 0};
+    holo_context::GsKey key{nx, ny, p.B, p.hb, p.conj_b, g.W, g.rows, Ex, Ey, 0, 0, 0};
     const int chunk = std::min(body, kGsGraphIters);
     hipGraphExec_t full = nullptr, rest = nullptr;
     HOLO_TRY(gs_graph(h, key, chunk, g, Ex, Ey, &p, sc, &full));
@@ -1631,14 +1634,17 @@ int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* j
     const Tuning gt = plane_tiles(h, nx, ny);
     HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &gt));
     const size_t plane = static_cast<size_t>(nx) * ny;
-    const size_t perpair = 3 * plane * sizeof(float2);
+    // per pair: field, forward and backward tables, copies of I1 and I2 (so that captured
+    // iterations reference workspace addresses only)
+    const size_t perpair = plane * (3 * sizeof(float2) + 2 * sizeof(float));
     const size_t chunk = std::max<size_t>(1, std::min<size_t>(kMaxJobs, budget / perpair));
     const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
     for (size_t c0 = g0; c0 < g1; c0 += chunk) {
       const size_t c1 = std::min(g1, c0 + chunk);
       HOLO_TRY(ensure_ws(h, perpair * (c1 - c0)));
       std::vector<GsPair> pairs(c1 - c0);
-      int maxit = 0;
+      int maxit = 0, minit = std::numeric_limits<int>::max();
+      unsigned long long conjmask = 0;
       for (size_t q = c0; q < c1; ++q) {
         const holo_gs_job& j = jobs[order[q]];
         GsPair& p = pairs[q - c0];
@@ -1646,18 +1652,45 @@ int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* j
         float2* hf = p.B + plane;
         p.hf = hf;
         HOLO_TRY(gs_transfers(g, &j.fwd, &j.bwd, hf, hf + plane, &p.hb, &p.conj_b, st));
-        p.I1 = j.holo1;
-        p.I2 = j.holo2;
+        float* i1c = reinterpret_cast<float*>(hf + 2 * plane);
+        float* i2c = i1c + plane;
+        HOLO_HIP(hipMemcpyAsync(i1c, j.holo1, plane * sizeof(float), hipMemcpyDeviceToDevice, st));
+        HOLO_HIP(hipMemcpyAsync(i2c, j.holo2, plane * sizeof(float), hipMemcpyDeviceToDevice, st));
+        p.I1 = i1c;
+        p.I2 = i2c;
         p.out = static_cast<float2*>(j.out);
         p.iters = j.iters;
         HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, j.holo1, 0, p.B, 0, 1.0f, 1, st));
         maxit = std::max(maxit, j.iters);
+        minit = std::min(minit, j.iters);
+        if (p.conj_b) conjmask |= 1ull << (q - c0);
       }
-      for (int it = 0; it < maxit; ++it)
+      // iterations 0 .. minit-2 are identical for the chunk (every pair active, none storing its
+      // result): graphs of up to kGsGraphIters of them, captured once per chunk layout; the rest
+      // (the last iteration of each pair, pairs with more iterations) are launched directly
+      const int body = h->gs_batch_graphs ? minit - 1 : 0;
+      if (body > 0) {
+        holo_context::GsKey key{nx, ny, h->ws, nullptr, 0, g.W, g.rows, Ex, Ey, 0,
+                                static_cast<int>(pairs.size()), conjmask};
+        const int gchunk = std::min(body, kGsGraphIters);
+        hipGraphExec_t full = nullptr, rest = nullptr;
+        const int np = static_cast<int>(pairs.size());
+        HOLO_TRY(gs_graph(h, key, gchunk, g, Ex, Ey, pairs.data(), sc, &full, np));
+        if (body % gchunk) HOLO_TRY(gs_graph(h, key, body % gchunk, g, Ex, Ey, pairs.data(), sc, &rest, np));
+        for (int c = 0; c < body / gchunk; ++c) HOLO_HIP(hipGraphLaunch(full, st));
+        if (rest) HOLO_HIP(hipGraphLaunch(rest, st));
+      }
+      for (int it = std::max(body, 0); it < maxit; ++it)
         HOLO_TRY(gs_round(h, g, Ex, Ey, pairs.data(), static_cast<int>(pairs.size()), it, sc, st));
     }
     g0 = g1;
   }
+  return HOLO_OK;
+}
+
+int holo_set_gs_graphs(holo_handle h, int enable) {
+  if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
+  h->gs_batch_graphs = enable != 0;
   return HOLO_OK;
 }
 

@@ -91,6 +91,7 @@ SIGNATURES = {
     "holo_get_tuning_for_size": [_vp, _i, _i, _i, ctypes.POINTER(ctypes.c_int)],
     "holo_autotune": [_vp, _vp, _i, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
     "holo_set_profiling": [_vp, _i],
+    "holo_set_gs_graphs": [_vp, _i],
     "holo_profile_read": [_vp, _i, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong),
                           ctypes.POINTER(ctypes.c_longlong), _i],
 }
@@ -169,6 +170,10 @@ class Handle:
         out = ctypes.c_size_t()
         check(self.lib.holo_workspace_size(self.ptr, ctypes.byref(out)))
         return out.value
+
+    def set_gs_graphs(self, enable: bool):
+        """Batched GS: run the chunk-uniform iterations as cached hipGraphs (default) or directly."""
+        check(self.lib.holo_set_gs_graphs(self.ptr, 1 if enable else 0))
 
     def set_profiling(self, enable: bool):
         check(self.lib.holo_set_profiling(self.ptr, 1 if enable else 0))

@@ -128,7 +128,11 @@ class Server:
     # that make the tail finish sooner; small requests wait a little longer.
     def __init__(self, device: int = 0, n_slots: int = 2, max_batch: int = 32,
                  hbm_budget: int = 32 << 30, workspace_per_slot: int = 2 << 30,
-                 max_group_work: Optional[int] = 32 << 30):
+                 max_group_work: Optional[int] = 32 << 30, keep_results: bool = True):
+        # keep_results: each request keeps its host projection (Request.proj); False hands the
+        # result over at completion and releases its pinned host buffer for reuse, so that a
+        # long stream holds pinned memory only for the groups in flight
+        self.keep_results = keep_results
         self.device = device
         self.dev = torch.device(f"cuda:{device}")
         self.max_batch = max_batch
@@ -189,7 +193,7 @@ class Server:
                 if g.event.query():
                     done = time.perf_counter() - t0
                     for r, hp in zip(g.reqs, g.outs[3]):
-                        r.proj = hp
+                        r.proj = hp if self.keep_results else None
                         r.t_done = done
                     used -= g.bytes
                     g.slot.busy = False

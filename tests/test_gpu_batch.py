@@ -168,6 +168,34 @@ def test_gs_batch_non_square_vs_oracle(gpu, nx, ny, iters):
     assert asm.weighted_phase_rms(out.cpu().numpy(), ref) < 1e-3
 
 
+def test_gs_batch_graphs_equal_direct_launches(gpu):
+    """The chunk-uniform iterations of batched GS run as cached hipGraphs: every replay (three
+    calls, with direct-launch calls, other device work and synchronizations between them, and
+    new intensities each time) equals the directly launched result bit for bit."""
+    from particleholography import _lib
+    from particleholography.batch import GSJob, phase_retrieval_batch
+
+    hd = _lib.handle(0)
+    specs = [(128, 37, 2000.0), (128, 21, 1500.0), (128, 40, 2500.0)]  # pairs with different counts
+    try:
+        for rep in range(3):
+            jobs = []
+            for k, (n, it, z) in enumerate(specs):
+                i1, i2 = _gs_pair(n, 300 + 10 * rep + k, z)
+                jobs.append(GSJob(torch.from_numpy(i1).to(gpu), torch.from_numpy(i2).to(gpu), z, it, LAM, DX))
+            hd.set_gs_graphs(False)
+            direct = [o.clone() for o in phase_retrieval_batch(jobs)]
+            torch.cuda.synchronize()
+            _ = torch.randn(1 << 20, device=gpu).sum().item()  # unrelated device work between replays
+            hd.set_gs_graphs(True)
+            graphed = phase_retrieval_batch(jobs)
+            torch.cuda.synchronize()
+            for a, b in zip(direct, graphed):
+                np.testing.assert_array_equal(b.cpu().numpy(), a.cpu().numpy())
+    finally:
+        hd.set_gs_graphs(True)
+
+
 def test_workspace_size_is_queryable(gpu):
     """holo_workspace_size reports the handle's device workspace, which holds at least the
     spectrum and one round of column-transformed slices after a reconstruction."""
