@@ -268,6 +268,12 @@ def _single_gs(h1, h2, r):
     return out
 
 
+def alone_gs(h1, h2, r):
+    """The request's phase retrieval computed by itself through the batched entry point the
+    server uses (the same tiles, so the served result equals it bit for bit)."""
+    return phase_retrieval_batch([GSJob(h1, h2, r.gs_z, r.gs_iters, r.wavlen, r.dx)], h1.device.index or 0)[0]
+
+
 def _single_recon(src, r):
     import ctypes
     ny, nx = src.shape

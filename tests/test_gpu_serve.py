@@ -19,7 +19,7 @@ def test_served_results_equal_results_alone(gpu):
     assert any(r.is_gs for r in reqs) and any(not r.is_gs for r in reqs)
     for r in reqs:
         h1 = r.holo.to(gpu)
-        src = serve._single_gs(h1, r.holo2.to(gpu), r) if r.is_gs else h1
+        src = serve.alone_gs(h1, r.holo2.to(gpu), r) if r.is_gs else h1
         alone = serve._single_recon(src, r).cpu().numpy()
         np.testing.assert_array_equal(r.proj.numpy(), alone)
 
@@ -69,7 +69,7 @@ def test_c5_mix_served_equals_alone_and_fp64_oracle(gpu):
     assert {tuple(r.holo.shape) for r in reqs} >= {(512, 512), (1024, 1024)}
     for r in reqs:
         h1 = r.holo.to(gpu)
-        src = serve._single_gs(h1, r.holo2.to(gpu), r) if r.is_gs else h1
+        src = serve.alone_gs(h1, r.holo2.to(gpu), r) if r.is_gs else h1
         alone = serve._single_recon(src, r).cpu().numpy()
         np.testing.assert_array_equal(r.proj.numpy(), alone)
     checked = 0
