@@ -1171,10 +1171,13 @@ int holo_make_background(holo_handle h, void* stream, int nx, int ny, int count,
     static_assert(2 * kModePairs == 4 * kMode8Quads, "both mode kernels take 128-pixel tiles");
     if (count <= 256) {  // 8-bit counters
       const bool vec4 = (npx % 4 == 0) && (reinterpret_cast<uintptr_t>(stack) % 4 == 0);
-      if (vec4)
-        hipLaunchKernelGGL(k_background_mode8<true>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
+      const bool buf = vec4 && npx * static_cast<size_t>(count) <= 0x7fffffffull;
+      if (buf)
+        hipLaunchKernelGGL(k_background_mode8<2>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
+      else if (vec4)
+        hipLaunchKernelGGL(k_background_mode8<1>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
       else
-        hipLaunchKernelGGL(k_background_mode8<false>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
+        hipLaunchKernelGGL(k_background_mode8<0>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
       return check_launch();
     }
     const int vec = (npx % 2 == 0) && (reinterpret_cast<uintptr_t>(stack) % 2 == 0);

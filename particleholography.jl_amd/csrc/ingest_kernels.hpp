@@ -141,82 +141,107 @@ __global__ __launch_bounds__(kModeThreads) void k_background_mode(size_t npx, in
 // of four consecutive pixels, so a 128-pixel tile needs 32 KB of LDS and four workgroups fit a
 // CU (twice the loads in flight of the 16-bit kernel). Thread (quad p = t % 32, phase
 // q = t / 32) counts images q, q+8, ... below 255, so no counter passes 255; the 256th image,
-// if any, is added in registers while phase q scans bins [32 q, 32 q + 32). VEC4: npx % 4 == 0
-// and the stack 4-byte aligned, one uchar4 load per quad and image.
+// if any, is added in registers while phase q scans bins [32 q, 32 q + 32).
+// LOAD 0: byte loads; 1 (npx % 4 == 0, stack 4-byte aligned): one uchar4 load per quad and
+// image; 2 (as 1, and the stack within 2 GiB): the same words through a buffer resource, the
+// image stride in the scalar offset and one 32-bit offset register per thread, all 32 of a
+// thread's images in flight at once (8 phases x 32 = the whole 256-image stack); images past
+// the stack read past the buffer's range and return 0 (their counts are skipped).
+// The LDS histogram bounds the pixels a CU can hold, so the images in flight per pixel set the
+// bytes in flight: 16 per thread gave 64 KB per CU (0.11 ms for 256 x 1024^2).
 constexpr int kMode8Quads = 32;
 constexpr int kMode8Phases = kModeThreads / kMode8Quads;  // 8
-constexpr int kMode8Unroll = 16;
-template <bool VEC4>
+template <int LOAD>
+constexpr int kMode8Unroll = LOAD == 2 ? 32 : 16;
+template <int LOAD>
 __global__ __launch_bounds__(kModeThreads) void k_background_mode8(size_t npx, int count,
                                                                     const uint8_t* __restrict__ stack,
                                                                     double* __restrict__ out) {
-  __shared__ unsigned int hist[256 * kMode8Quads];
+  constexpr bool VEC4 = LOAD >= 1;
+  constexpr int kUnroll = kMode8Unroll<LOAD>;
+  __shared__ __align__(16) unsigned int hist[256 * kMode8Quads];
   __shared__ unsigned int best[kMode8Phases][4 * kMode8Quads];
   const int t = threadIdx.x;
-  for (int k = t; k < 256 * kMode8Quads; k += kModeThreads) hist[k] = 0u;
+  for (int k = t; k < 256 * kMode8Quads / 4; k += kModeThreads) reinterpret_cast<uint4*>(hist)[k] = make_uint4(0u, 0u, 0u, 0u);
   __syncthreads();
   const int p = t % kMode8Quads;
   const int q = t / kMode8Quads;
   const size_t i0 = static_cast<size_t>(blockIdx.x) * (4 * kMode8Quads) + 4 * p;
-  bool has[4];
   size_t ix[4];
+  unsigned int inc[4];  // the counter increment of each pixel of the quad, 0 past the image
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    has[c] = i0 + c < npx;
-    ix[c] = has[c] ? i0 + c : 0;  // loads read valid addresses without a branch each
+    const bool has = i0 + c < npx;
+    ix[c] = has ? i0 + c : 0;  // loads read valid addresses without a branch each
+    inc[c] = has ? 1u << (8 * c) : 0u;
   }
   const int counted = min(count, 255);
-  for (int m0 = q; m0 < counted; m0 += kMode8Phases * kMode8Unroll) {
-    uchar4 w[kMode8Unroll];
+  for (int m0 = q; m0 < counted; m0 += kMode8Phases * kUnroll) {
+    uchar4 w[kUnroll];
+    if constexpr (LOAD == 2) {
+      const __amdgpu_buffer_rsrc_t r =
+          __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(stack), 0, static_cast<int>(count * npx), 0x00020000);
+      const unsigned voff = static_cast<unsigned>(static_cast<size_t>(m0) * npx + ix[0]);
+      const unsigned stride = static_cast<unsigned>(kMode8Phases * npx);
 #pragma unroll
-    for (int u = 0; u < kMode8Unroll; ++u) {
-      const uint8_t* s = stack + static_cast<size_t>(min(m0 + kMode8Phases * u, counted - 1)) * npx;
-      if constexpr (VEC4) {
-        w[u] = *reinterpret_cast<const uchar4*>(s + ix[0]);  // i0 % 4 == 0: has[0] implies has[3]
-      } else {
-        w[u] = make_uchar4(s[ix[0]], s[ix[1]], s[ix[2]], s[ix[3]]);
+      for (int u = 0; u < kUnroll; ++u)
+        w[u] = __builtin_bit_cast(uchar4, __builtin_amdgcn_raw_buffer_load_b32(r, voff, u * stride, 0));
+    } else {
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const uint8_t* s = stack + static_cast<size_t>(min(m0 + kMode8Phases * u, counted - 1)) * npx;
+        if constexpr (VEC4) {
+          w[u] = *reinterpret_cast<const uchar4*>(s + ix[0]);  // i0 % 4 == 0: has[0] implies has[3]
+        } else {
+          w[u] = make_uchar4(s[ix[0]], s[ix[1]], s[ix[2]], s[ix[3]]);
+        }
       }
     }
+    // pixels past the image add zero (no branch per update); images past the stack are
+    // skipped by a branch that is uniform over each half-wave (one phase)
 #pragma unroll
-    for (int u = 0; u < kMode8Unroll; ++u) {
+    for (int u = 0; u < kUnroll; ++u) {
       if (m0 + kMode8Phases * u < counted) {
-        if (has[0]) atomicAdd(&hist[w[u].x * kMode8Quads + p], 1u);
-        if (has[1]) atomicAdd(&hist[w[u].y * kMode8Quads + p], 1u << 8);
-        if (has[2]) atomicAdd(&hist[w[u].z * kMode8Quads + p], 1u << 16);
-        if (has[3]) atomicAdd(&hist[w[u].w * kMode8Quads + p], 1u << 24);
+        atomicAdd(&hist[w[u].x * kMode8Quads + p], inc[0]);
+        atomicAdd(&hist[w[u].y * kMode8Quads + p], inc[1]);
+        atomicAdd(&hist[w[u].z * kMode8Quads + p], inc[2]);
+        atomicAdd(&hist[w[u].w * kMode8Quads + p], inc[3]);
       }
     }
-  }
-  // the 256th image (count == 256), added during the scan
-  unsigned last[4] = {256u, 256u, 256u, 256u};  // 256: no bin
-  if (count > 255) {
-    const uint8_t* s = stack + static_cast<size_t>(255) * npx;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) last[c] = s[ix[c]];
   }
   __syncthreads();
-  unsigned cb[4] = {0u, 0u, 0u, 0u}, bb[4] = {0u, 0u, 0u, 0u};
+  // Phase q scans bins [32 q, 32 q + 32) of its quad. Each count and bin form the key
+  // count << 8 | (255 - bin): the largest key is the most frequent value, the smallest bin on
+  // ties (the reference's first strict maximum). One byte permute builds the 16-bit keys of two
+  // pixels from a counter word and the replicated 255 - bin, one packed 16-bit max keeps them.
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  u16x2 a01 = {0, 0}, a23 = {0, 0};
   for (int b = 32 * q; b < 32 * q + 32; ++b) {
     const unsigned int w = hist[b * kMode8Quads + p];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const unsigned n = ((w >> (8 * c)) & 0xffu) + (last[c] == static_cast<unsigned>(b) ? 1u : 0u);
-      if (n > cb[c]) { cb[c] = n; bb[c] = b; }
-    }
+    const unsigned int kk = (255u - static_cast<unsigned>(b)) * 0x01010101u;
+    // bytes of {kk, w}: 0-3 the counters of pixels 0-3, 4-7 copies of 255 - bin
+    const unsigned int k01 = __builtin_amdgcn_perm(kk, w, 0x01040004u);  // (c1:k, c0:k)
+    const unsigned int k23 = __builtin_amdgcn_perm(kk, w, 0x03040204u);  // (c3:k, c2:k)
+    a01 = __builtin_elementwise_max(a01, __builtin_bit_cast(u16x2, k01));
+    a23 = __builtin_elementwise_max(a23, __builtin_bit_cast(u16x2, k23));
   }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) best[q][4 * p + c] = (cb[c] << 8) | bb[c];
+  best[q][4 * p + 0] = a01.x;
+  best[q][4 * p + 1] = a01.y;
+  best[q][4 * p + 2] = a23.x;
+  best[q][4 * p + 3] = a23.y;
   __syncthreads();
   if (t < 4 * kMode8Quads) {
     const size_t i = static_cast<size_t>(blockIdx.x) * (4 * kMode8Quads) + t;
-    if (i < npx) {
-      unsigned int c = 0, bv = 0;
-      for (int r = 0; r < kMode8Phases; ++r) {  // earlier phases hold smaller bins: strict > keeps them
-        const unsigned int e = best[r][t];
-        if ((e >> 8) > c) { c = e >> 8; bv = e & 0xffu; }
-      }
-      out[i] = static_cast<double>(bv) / 255.0;
+    unsigned int key = 0;
+    for (int r = 0; r < kMode8Phases; ++r) key = max(key, best[r][t]);
+    if (count > 255) {
+      // the 256th image adds one to its own bin L only: the maximum over the updated counts is
+      // the larger of the maximum so far and L's updated key (a count of 256 needs the 9th bit)
+      const unsigned L = stack[static_cast<size_t>(255) * npx + (i < npx ? i : npx - 1)];
+      const unsigned cL = (hist[L * kMode8Quads + t / 4] >> (8 * (t % 4))) & 0xffu;
+      key = max(key, ((cL + 1u) << 8) | (255u - L));
     }
+    if (i < npx) out[i] = static_cast<double>(255u - (key & 0xffu)) / 255.0;
   }
 }
 

@@ -27,7 +27,8 @@ HOT = {
     "GS rows, 1024": "_ZN4holo9k_gs_rowsILi4ELi1024E",
     "GS rows, 2048": "_ZN4holo9k_gs_rowsILi8ELi2048E",
     "background mode": "_ZN4holo17k_background_modeILb1E",
-    "background mode, 8-bit counters": "_ZN4holo18k_background_mode8ILb1E",
+    "background mode, 8-bit counters": "_ZN4holo18k_background_mode8ILi1E",
+    "background mode, 8-bit counters, buffer loads": "_ZN4holo18k_background_mode8ILi2E",
     "ingest with background": "_ZN4holo11k_ingest_u8ILb1ELb1E",
 }
 
