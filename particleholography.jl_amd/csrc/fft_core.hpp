@@ -417,7 +417,7 @@ struct CPlan {
   static constexpr int T = L / E;
 };
 
-template <int R, int E, int DIR, int T, int P, int OFF, bool LAST, int PS, bool WAVE>
+template <int R, int E, int DIR, int T, int P, int OFF, bool LAST, int PS, bool WAVE, bool DB>
 __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const float2* __restrict__ tw,
                                              float2* __restrict__ lds, const int es) {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
@@ -447,7 +447,9 @@ __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const 
     for (int m = 0; m < R; ++m) v[q + Q * m] = x[m];
   }
   if constexpr (!LAST) {
-    lds_sync<WAVE>();  // every thread of the line has read the previous stage's LDS contents
+    // every thread of the line has read the buffer's previous contents (with two buffers the
+    // barrier after the previous exchange's writes already ordered those reads)
+    if constexpr (!DB) lds_sync<WAVE>();
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
@@ -456,20 +458,30 @@ __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const 
   }
 }
 
-template <int E, int DIR, int L, int PS, bool WAVE = false, int S = 0>
+// DB: exchanges alternate between `lds` and `lds + bo` (exchange s uses buffer s mod 2), so a
+// buffer is rewritten only after a barrier that follows every read of it. With an even number
+// of exchanges per line, consecutive lines of one loop keep that order too.
+template <int E, int DIR, int L, int PS, bool WAVE = false, int S = 0, bool DB = false>
 __device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__ lds, const int es,
-                                            const int t, const float2* __restrict__ tw) {
+                                            const int t, const float2* __restrict__ tw, const int bo = 0) {
   constexpr CPlanData d = CPlan<L, E>::d;
   if constexpr (S < d.ns) {
     constexpr int T = CPlan<L, E>::T;
     if constexpr (S > 0) {
+      const float2* __restrict__ rd = DB && ((S - 1) & 1) ? lds + bo : lds;
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = lds[lds_pad<PS>(t + T * e) * es];
+      for (int e = 0; e < E; ++e) v[e] = rd[lds_pad<PS>(t + T * e) * es];
     }
-    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns, PS, WAVE>(v, t, tw, lds, es);
-    fft_line_ct<E, DIR, L, PS, WAVE, S + 1>(v, lds, es, t, tw);
+    float2* __restrict__ wr = DB && (S & 1) ? lds + bo : lds;
+    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns, PS, WAVE, DB>(v, t, tw, wr, es);
+    fft_line_ct<E, DIR, L, PS, WAVE, S + 1, DB>(v, lds, es, t, tw, bo);
   }
 }
+
+// Column plans that can run double-buffered in a slice loop: compile-time, complete, and an
+// even number of exchanges (an odd stage count).
+template <int L, int E>
+constexpr bool kColDB = L != 0 && CPlan<L, E>::d.complete && (CPlan<L, E>::d.ns % 2) == 1;
 
 // Line FFT with a compile-time plan when L != 0, else the runtime plan `pl`. WAVE: the line
 // lies within one wavefront (wave-scope ordering instead of workgroup barriers).
@@ -497,6 +509,13 @@ template <int E, int DIR, int L>
 __device__ __forceinline__ void fft_col(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,
                                         const LinePlan& pl, const float2* __restrict__ tw) {
   fft_run<E, DIR, L, kColPad>(v, col, W, t, pl, tw);
+}
+// A column line with two exchange buffers, the second `bo` elements after the first.
+template <int E, int DIR, int L>
+__device__ __forceinline__ void fft_col_db(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,
+                                           const float2* __restrict__ tw, const int bo) {
+  static_assert(kColDB<L, E>, "double-buffered columns need a compile-time plan with an even number of exchanges");
+  fft_line_ct<E, DIR, L, kColPad, false, 0, true>(v, col, W, t, tw, bo);
 }
 
 // Threads per line: a constant when the plan is compile-time.

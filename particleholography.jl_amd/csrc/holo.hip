@@ -58,6 +58,7 @@ struct Plan1D {
 struct Tuning {
   int batch = 0, spg_gen = 0, spg_out = 0, cols = 0, rows = 0;
   int erow = 0, ecol = 0;  // elements per thread of row / column passes (0: the length's default)
+  int col_db = 0;          // pass A exchange buffers: 1 two, -1 one, 0 two where occupancy allows
 };
 
 // E = elements per thread; the set below is what the kernels are instantiated for.
@@ -115,6 +116,8 @@ struct holo_context {
   int max_lds = 65536;
   std::map<int, Plan1D> plans;  // key (L << 8) | E
   int col_e = 0;                 // elements per thread of column passes (0: the length's default)
+  int col_db = 1;                // 0 (HOLO_COL_DB=0): pass A always with one exchange buffer
+  std::map<std::tuple<const void*, int, size_t>, int> occ;  // resident workgroups per CU, by kernel
   void* ws = nullptr;
   size_t ws_bytes = 0;
   Tuning tune;                                    // global override (tests, sweeps)
@@ -300,8 +303,8 @@ int ilog2(int v) {
 size_t row_lds(const Geom& g) {
   return (static_cast<size_t>(g.rows) * lds_padded_len(g.nx, kRowPad) + g.px.ntw) * sizeof(float2);
 }
-size_t col_lds(const Geom& g) {
-  return (static_cast<size_t>(g.W) * lds_padded_len(g.ny, kColPad) + g.py.ntw) * sizeof(float2);
+size_t col_lds(const Geom& g, int nbuf = 1) {
+  return (static_cast<size_t>(nbuf) * g.W * lds_padded_len(g.ny, kColPad) + g.py.ntw) * sizeof(float2);
 }
 
 // Effective tuning for a size: the per-size entry, with every nonzero field of the global
@@ -315,6 +318,7 @@ Tuning eff_tuning(const holo_context* h, int nx, int ny) {
   if (h->tune.spg_out > 0) t.spg_out = h->tune.spg_out;
   if (h->tune.cols > 0) t.cols = h->tune.cols;
   if (h->tune.rows > 0) t.rows = h->tune.rows;
+  if (h->tune.col_db != 0) t.col_db = h->tune.col_db;
   return t;
 }
 
@@ -447,6 +451,20 @@ void allow_lds(K kernel, size_t bytes) {
   if (bytes > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+}
+
+// Workgroups of `kernel` resident per CU at `block` threads and `lds` bytes (cached per handle).
+template <class K>
+int resident(holo_context* h, K kernel, int block, size_t lds) {
+  const auto key = std::make_tuple(reinterpret_cast<const void*>(kernel), block, lds);
+  auto it = h->occ.find(key);
+  if (it != h->occ.end()) return it->second;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(kernel), block, lds) !=
+      hipSuccess)
+    n = 0;
+  h->occ.emplace(key, n);
+  return n;
 }
 
 // ---- launch helpers ---------------------------------------------------------------------
@@ -710,12 +728,29 @@ int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp,
   ra.spg = rp.spg_gen;
   ra.gp.wavlen = wavlen;
   ra.gp.dx = dx;
-  const size_t lds = col_lds(g);
+  const size_t lds = col_lds(g), lds2 = col_lds(g, 2);
+  const Tuning tn = eff_tuning(h, g.nx, g.ny);
   bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
     dim3 grid = col_grid(g, (maxnb + rp.spg_gen - 1) / rp.spg_gen);
     grid.z = jt.n;
+    // two exchange buffers when the plan allows it, the tile's LDS fits twice and the size's
+    // tuning asks for them (by default: when they leave as many workgroups resident per CU)
+    if constexpr (kColDB<L, E>) {
+      bool db = h->col_db && tn.col_db >= 0 && lds2 <= static_cast<size_t>(h->max_lds);
+      if (db && tn.col_db == 0) {
+        allow_lds(k_recon_cols<E, L, 0, true>, lds2);
+        allow_lds(k_recon_cols<E, L, 0>, lds);
+        const int bs = static_cast<int>(col_block(g).x);
+        db = resident(h, k_recon_cols<E, L, 0, true>, bs, lds2) >= resident(h, k_recon_cols<E, L, 0>, bs, lds);
+      }
+      if (db) {
+        allow_lds(k_recon_cols<E, L, 0, true>, lds2);
+        hipLaunchKernelGGL((k_recon_cols<E, L, 0, true>), grid, col_block(g), lds2, st, ra, jt, g);
+        return;
+      }
+    }
     allow_lds(k_recon_cols<E, L, 0>, lds);
     hipLaunchKernelGGL((k_recon_cols<E, L, 0>), grid, col_block(g), lds, st, ra, jt, g);
   });
@@ -946,6 +981,7 @@ int holo_create(holo_handle* out, int device) {
   const size_t cap = env_size("HOLO_MAX_LDS", 0);
   if (cap > 0) h->max_lds = static_cast<int>(std::min<size_t>(cap, h->max_lds));
   h->col_e = static_cast<int>(env_size("HOLO_COL_E", 0));
+  h->col_db = static_cast<int>(env_size("HOLO_COL_DB", 1));
   // Per-size tiles chosen by holo_autotune on an MI355X and confirmed on single-hologram
   // reconstructions (768^2 and 1280x1024 +37%, 512^2 +5% over the generic rule). 2048^2 comes
   // from a direct sweep: no useful round fits the 256 MB Infinity Cache at that size, so long
@@ -953,12 +989,12 @@ int holo_create(holo_handle* out, int device) {
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
-      int nx, ny, batch, spg_gen, spg_out, cols, rows, erow = 0, ecol = 0;
+      int nx, ny, batch, spg_gen, spg_out, cols, rows, erow = 0, ecol = 0, col_db = 0;
     };
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
                                        {768, 768, 56, 16, 4, 4, 2},
                                        {1280, 1024, 25, 8, 8, 4, 4},
-                                       {2048, 2048, 64, 16, 16, 4, 2}};
+                                       {2048, 2048, 64, 16, 16, 4, 2, 0, 0, -1}};
     for (const Shipped& e : kShipped) {
       Tuning t;
       t.batch = e.batch;
@@ -968,6 +1004,7 @@ int holo_create(holo_handle* out, int device) {
       t.rows = e.rows;
       t.erow = e.erow;
       t.ecol = e.ecol;
+      t.col_db = e.col_db;
       h->size_tune[{e.nx, e.ny}] = t;
     }
   }
@@ -1903,17 +1940,20 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     if ((bi >= 1 && t.batch == tried[0]) || (bi == 2 && t.batch == tried[1])) continue;
     double ba = 1e30, bb = 1e30;
     Tuning ta = t, tb = t;
-    // pass A (columns, generates H_k): columns per workgroup and slices per workgroup
+    // pass A (columns, generates H_k): columns per workgroup, slices per workgroup and one or
+    // two exchange buffers
     for (int W : {2, 4, 8, 16})
-      for (int sg : {4, 8, 16}) {
-        Tuning c = t;
-        c.cols = W;
-        c.spg_gen = std::min(sg, t.batch);
-        c.spg_out = 8;
-        double v;
-        if (time_round(c, true, false, &v) != HOLO_OK) continue;
-        if (v < ba) { ba = v; ta = c; }
-      }
+      for (int sg : {4, 8, 16})
+        for (int db : {-1, 1}) {
+          Tuning c = t;
+          c.cols = W;
+          c.spg_gen = std::min(sg, t.batch);
+          c.spg_out = 8;
+          c.col_db = db;
+          double v;
+          if (time_round(c, true, false, &v) != HOLO_OK) continue;
+          if (v < ba) { ba = v; ta = c; }
+        }
     // pass B (rows, forms the projection): rows per workgroup and slices per workgroup
     for (int rows : {1, 2, 4, 8})
       for (int so : {4, 8, 16}) {

@@ -269,14 +269,15 @@ __device__ __forceinline__ void col_store_buf(const float2 (&v)[E], __amdgpu_buf
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[e]), r, voff, e * row_bytes, 0);
 }
 
-template <int E, int L, bool STAGE = true>
+// NBUF exchange buffers of the tile's W columns, then the twiddle table.
+template <int E, int L, bool STAGE = true, int NBUF = 1>
 __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   ColCtx c;
   c.w = threadIdx.x & (g.W - 1);
   c.t = threadIdx.x >> g.wshift;
   c.i = xcd_tile(blockIdx.x, gridDim.x) * g.W + c.w;
   c.col = lds + c.w;
-  float2* tw = lds + g.W * lds_padded_len(g.ny, kColPad);
+  float2* tw = lds + NBUF * g.W * lds_padded_len(g.ny, kColPad);
   if (STAGE) stage_twiddles(tw, g.twy, g.py.ntw);
   c.tw = tw;
   return c;
@@ -388,14 +389,17 @@ __device__ __forceinline__ size_t centred_index(int i, int j, int nx, int ny) {
   return static_cast<size_t>(jc) * nx + ic;
 }
 
-template <int E, int L, int HMODE>
+// DB (HMODE 0 only): the slice loop's column FFTs alternate between two exchange buffers,
+// two workgroup barriers per slice instead of four.
+template <int E, int L, int HMODE, bool DB = false>
 __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom g) {
+  static_assert(!DB || HMODE == 0, "double buffering is for the generated-H slice loop");
   extern __shared__ float2 lds[];
   // HMODE 0: hologram blockIdx.z of the table; workgroups past its slice count leave at once
   const JobDesc& jd = jt.d[HMODE == 0 ? blockIdx.z : 0];
   if (HMODE == 0 && static_cast<int>(blockIdx.y) * a.spg >= jd.nbatch) return;
   const float2* __restrict__ Fsrc = HMODE == 0 ? jd.F : a.F;
-  const ColCtx c = col_ctx<E, L, false>(g, lds);
+  const ColCtx c = col_ctx<E, L, false, DB ? 2 : 1>(g, lds);
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
@@ -445,7 +449,11 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
 #pragma unroll
         for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
       }
-      fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+      if constexpr (DB) {
+        fft_col_db<E, +1, L>(v, c.col, g.W, c.t, c.tw, g.W * lds_padded_len(g.ny, kColPad));
+      } else {
+        fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+      }
       // columns past the plane (the last tile) get an offset past the buffer: dropped stores
       col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? off * 8u : pbytes, rowb);
     }
