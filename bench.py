@@ -231,7 +231,8 @@ def main():
         extras["ingest_1024"] = bench_ingest(ph, torch, dev)
         extras["detect_reference_holograms"] = bench_detect(ph, torch, dev)
         extras["batched_64"] = bench_batched(torch, dev, args.batch_holograms)
-        extras["serving"] = bench_serving(torch, args.serve_requests, plain)
+        if args.serve_requests > 0:
+            extras["serving"] = bench_serving(torch, args.serve_requests, plain)
 
     cpu, parity = None, None
     if not args.no_cpu_baseline:

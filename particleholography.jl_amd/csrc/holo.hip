@@ -59,6 +59,7 @@ struct Tuning {
   int batch = 0, spg_gen = 0, spg_out = 0, cols = 0, rows = 0;
   int erow = 0, ecol = 0;  // elements per thread of row / column passes (0: the length's default)
   int col_db = 0;          // pass A exchange buffers: 1 two, -1 one, 0 two where occupancy allows
+  int row_pf = 0;          // pass B rows loaded one slice ahead: 1 yes, -1 / 0 no
 };
 
 // E = elements per thread; the set below is what the kernels are instantiated for.
@@ -117,6 +118,7 @@ struct holo_context {
   std::map<int, Plan1D> plans;  // key (L << 8) | E
   int col_e = 0;                 // elements per thread of column passes (0: the length's default)
   int col_db = 1;                // 0 (HOLO_COL_DB=0): pass A always with one exchange buffer
+  int row_pf = 1;                // pass B rows one slice ahead: HOLO_ROW_PF 0 never, 1 per size, 2 always
   std::map<std::tuple<const void*, int, size_t>, int> occ;  // resident workgroups per CU, by kernel
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -319,6 +321,7 @@ Tuning eff_tuning(const holo_context* h, int nx, int ny) {
   if (h->tune.cols > 0) t.cols = h->tune.cols;
   if (h->tune.rows > 0) t.rows = h->tune.rows;
   if (h->tune.col_db != 0) t.col_db = h->tune.col_db;
+  if (h->tune.row_pf != 0) t.row_pf = h->tune.row_pf;
   return t;
 }
 
@@ -560,6 +563,35 @@ ReconPlan recon_plan(holo_context* h, const Geom& g, int slices) {
 int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan& rp, double wavlen,
                  double dx, const JobTable& jt, int maxnb, hipStream_t st, bool do_a = true,
                  bool do_b = true, bool timed = true);
+
+// Pass B of a projection round: k_recon_rows_proj over (row tiles, `groups` slice groups,
+// holograms of `jt`); rows prefetched one slice ahead where the size's tuning asks for it and the
+// plane fits a 32-bit buffer range.
+int launch_rows_proj(holo_context* h, const Geom& g, int Ex, const JobTable& jt, int spg, int groups,
+                     hipStream_t st) {
+  const size_t plane = static_cast<size_t>(g.nx) * g.ny;
+  const bool pf = (h->row_pf == 2 || (h->row_pf == 1 && eff_tuning(h, g.nx, g.ny).row_pf > 0)) &&
+                  plane * sizeof(float2) <= 0xffffffffull;
+  const size_t rlds = row_lds(g);
+  const float sc2 = proj_scale(g.nx, g.ny);
+  dim3 grid = row_grid(g, groups);
+  grid.z = jt.n;
+  bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
+    constexpr int E = decltype(ec)::value;
+    constexpr int L = decltype(lc)::value;
+    if constexpr (L != 0) {
+      if (pf) {
+        allow_lds(k_recon_rows_proj<E, L, true>, rlds);
+        hipLaunchKernelGGL((k_recon_rows_proj<E, L, true>), grid, row_block(g), rlds, st, jt, plane, spg, sc2, g);
+        return;
+      }
+    }
+    allow_lds(k_recon_rows_proj<E, L>, rlds);
+    hipLaunchKernelGGL((k_recon_rows_proj<E, L>), grid, row_block(g), rlds, st, jt, plane, spg, sc2, g);
+  });
+  if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
+  return check_launch();
+}
 int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp, double wavlen, double dx,
                     const JobTable& jt, int maxnb, hipStream_t st);
 
@@ -630,7 +662,6 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     return HOLO_OK;
   }
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
-  const float sc2 = proj_scale(nx, ny);
   const size_t clds = col_lds(g), rlds = row_lds(g);
   for (int b0 = 0; b0 < slices; b0 += rp.m) {
     const int nb = std::min(rp.m, slices - b0);
@@ -671,13 +702,15 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
     prof_end(h, st, pa, HOLO_PASS_RECON_COLS, nb);
     const int ry = (nb + rp.spg_out - 1) / rp.spg_out;
     hipEvent_t pb = fixed ? nullptr : prof_begin(h, st);
+    if (kind == 0) {
+      HOLO_TRY(launch_rows_proj(h, g, Ex, jt, rp.spg_out, ry, st));
+      prof_end(h, st, pb, HOLO_PASS_RECON_ROWS, nb);
+      continue;
+    }
     bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
       constexpr int E = decltype(ec)::value;
-    constexpr int L = decltype(lc)::value;
-      if (kind == 0) {
-        allow_lds(k_recon_rows_proj<E, L>, rlds);
-        hipLaunchKernelGGL((k_recon_rows_proj<E, L>), row_grid(g, ry), row_block(g), rlds, st, jt, plane, rp.spg_out, sc2, g);
-      } else {
+      constexpr int L = decltype(lc)::value;
+      {
         StoreArgs sa{};
         sa.G = G;
         sa.gstride = plane;
@@ -764,8 +797,6 @@ int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp,
 // per-hologram slice count of the round.
 int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan& rp, double wavlen,
                  double dx, const JobTable& jt, int maxnb, hipStream_t st, bool do_a, bool do_b, bool timed) {
-  const size_t plane = static_cast<size_t>(g.nx) * g.ny;
-  const float sc2 = proj_scale(g.nx, g.ny);
   long long slices = 0;
   for (int i = 0; i < jt.n; ++i) slices += jt.d[i].nbatch;
   if (do_a) {
@@ -774,18 +805,8 @@ int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan
     prof_end(h, st, pa, HOLO_PASS_RECON_COLS, slices);
   }
   if (do_b) {
-    const size_t rlds = row_lds(g);
     hipEvent_t pb = timed ? prof_begin(h, st) : nullptr;
-    bool ok = dispatch_EL(Ex, g.nx, [&](auto ec, auto lc) {
-      constexpr int E = decltype(ec)::value;
-      constexpr int L = decltype(lc)::value;
-      allow_lds(k_recon_rows_proj<E, L>, rlds);
-      dim3 grid = row_grid(g, (maxnb + rp.spg_out - 1) / rp.spg_out);
-      grid.z = jt.n;
-      hipLaunchKernelGGL((k_recon_rows_proj<E, L>), grid, row_block(g), rlds, st, jt, plane, rp.spg_out, sc2, g);
-    });
-    if (!ok) return fail(HOLO_ERR_SIZE, "no row kernel for this size");
-    HOLO_TRY(check_launch());
+    HOLO_TRY(launch_rows_proj(h, g, Ex, jt, rp.spg_out, (maxnb + rp.spg_out - 1) / rp.spg_out, st));
     prof_end(h, st, pb, HOLO_PASS_RECON_ROWS, slices);
   }
   return HOLO_OK;
@@ -982,18 +1003,23 @@ int holo_create(holo_handle* out, int device) {
   if (cap > 0) h->max_lds = static_cast<int>(std::min<size_t>(cap, h->max_lds));
   h->col_e = static_cast<int>(env_size("HOLO_COL_E", 0));
   h->col_db = static_cast<int>(env_size("HOLO_COL_DB", 1));
+  h->row_pf = static_cast<int>(env_size("HOLO_ROW_PF", 1));
   // Per-size tiles chosen by holo_autotune on an MI355X and confirmed on single-hologram
   // reconstructions (768^2 and 1280x1024 +37%, 512^2 +5% over the generic rule). 2048^2 comes
   // from a direct sweep: no useful round fits the 256 MB Infinity Cache at that size, so long
-  // rounds (64 slices, 2 GB) amortise each launch's tail (53.8k slices/s against 48.9k with 16);
+  // rounds (64 slices, 2 GB) amortise each launch's tail (53.8k slices/s against 48.9k with 16),
+  // and its pass A keeps one exchange buffer (two: 3.88 ms against 3.76 per 200 slices). Pass B
+  // loads rows one slice ahead at 1024^2 (3.34 ms against 3.46 per 1000 slices) and 1280x1024
+  // (2.86 against 3.17 per 500); at 512^2, 768^2 and 2048^2 it was 0.2-1.4% slower.
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
-      int nx, ny, batch, spg_gen, spg_out, cols, rows, erow = 0, ecol = 0, col_db = 0;
+      int nx, ny, batch, spg_gen, spg_out, cols, rows, erow = 0, ecol = 0, col_db = 0, row_pf = 0;
     };
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
+                                       {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 0, 1},
                                        {768, 768, 56, 16, 4, 4, 2},
-                                       {1280, 1024, 25, 8, 8, 4, 4},
+                                       {1280, 1024, 25, 8, 8, 4, 4, 0, 0, 0, 1},
                                        {2048, 2048, 64, 16, 16, 4, 2, 0, 0, -1}};
     for (const Shipped& e : kShipped) {
       Tuning t;
@@ -1005,6 +1031,7 @@ int holo_create(holo_handle* out, int device) {
       t.erow = e.erow;
       t.ecol = e.ecol;
       t.col_db = e.col_db;
+      t.row_pf = e.row_pf;
       h->size_tune[{e.nx, e.ny}] = t;
     }
   }
@@ -1954,16 +1981,19 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
           if (time_round(c, true, false, &v) != HOLO_OK) continue;
           if (v < ba) { ba = v; ta = c; }
         }
-    // pass B (rows, forms the projection): rows per workgroup and slices per workgroup
+    // pass B (rows, forms the projection): rows per workgroup, slices per workgroup and rows
+    // loaded one slice ahead or not
     for (int rows : {1, 2, 4, 8})
-      for (int so : {4, 8, 16}) {
-        Tuning c = ta;
-        c.rows = rows;
-        c.spg_out = std::min(so, t.batch);
-        double v;
-        if (time_round(c, false, true, &v) != HOLO_OK) continue;
-        if (v < bb) { bb = v; tb = c; }
-      }
+      for (int so : {4, 8, 16})
+        for (int pf : {-1, 1}) {
+          Tuning c = ta;
+          c.rows = rows;
+          c.spg_out = std::min(so, t.batch);
+          c.row_pf = pf;
+          double v;
+          if (time_round(c, false, true, &v) != HOLO_OK) continue;
+          if (v < bb) { bb = v; tb = c; }
+        }
     if (ba + bb < best_total) {
       best_total = ba + bb;
       best = tb;

@@ -48,8 +48,10 @@ __device__ __forceinline__ double axis_a(int n, int N, double wavlen, double dx)
 // most two FP64 ulps, a phase change below 1e-10 cycles at |z| = 2e5 um. The two FP64
 // divisions per element of the reference form were a fifth of pass A's instructions.
 __device__ __forceinline__ double axis_scale(int N, double wavlen, double dx) { return wavlen / N / dx; }
+// n in [0, N): the centring shift is one conditional subtraction (no integer division).
 __device__ __forceinline__ double axis_a_s(int n, int N, double s) {
-  const int ic = (n + (N >> 1)) % N;
+  int ic = n + (N >> 1);
+  if (ic >= N) ic -= N;
   const double t = (static_cast<double>(ic) - 0.5 * N) * s;
   return t * t;
 }
@@ -519,7 +521,10 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, size_t n, unsigned int 
 // non-negative floats, NaN above them, as in the merge below) and scales it once by
 // sc2 = 1/(nx*ny)^2 before the merge; for power-of-two sizes the scaling is exact and the result
 // equals the per-slice scaled form bit for bit.
-template <int E, int L>
+// PF: the next slice's row is loaded while the current one is transformed (two row buffers
+// used alternately, two slices per loop iteration), so a wave waits for memory once per
+// workgroup instead of once per slice.
+template <int E, int L, bool PF = false>
 __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gstride, int spg,
                                                          float sc2, Geom g) {
   extern __shared__ float2 lds[];
@@ -536,14 +541,39 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
   unsigned mn[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) mn[e] = 0x7fffffffu;
-  for (int k = k0; k < k1; ++k) {
-    float2 v[E];
-    const float2* __restrict__ src = G + static_cast<size_t>(k) * gstride + rowoff;
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = src[c.t + T * e];
+  auto fold = [&](float2 (&v)[E]) {
     fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) mn[e] = min(mn[e], __float_as_uint(v[e].x * v[e].x + v[e].y * v[e].y));
+  };
+  if constexpr (PF) {
+    // rows through a buffer resource: the slice's base in SGPRs, one 32-bit offset per thread
+    const unsigned pbytes = static_cast<unsigned>(gstride * sizeof(float2));
+    const unsigned voff = static_cast<unsigned>((rowoff + c.t) * sizeof(float2));
+    auto load = [&](float2 (&v)[E], int k) {
+      const __amdgpu_buffer_rsrc_t r = plane_rsrc(G + static_cast<size_t>(k) * gstride, pbytes);
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        v[e] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, e * T * 8, 0));
+    };
+    float2 a[E], b[E];
+    load(a, k0);
+    int k = k0;
+    for (; k + 1 < k1; k += 2) {
+      load(b, k + 1);
+      fold(a);
+      if (k + 2 < k1) load(a, k + 2);
+      fold(b);
+    }
+    if (k < k1) fold(a);
+  } else {
+    for (int k = k0; k < k1; ++k) {
+      float2 v[E];
+      const float2* __restrict__ src = G + static_cast<size_t>(k) * gstride + rowoff;
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = src[c.t + T * e];
+      fold(v);
+    }
   }
   if (c.ok && k1 > k0) {
 #pragma unroll
