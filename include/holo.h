@@ -293,7 +293,9 @@ int holo_workspace_size(holo_handle h, size_t* bytes);
  * a server reserves its slots' workspaces up front so that no call does that mid-stream. */
 int holo_reserve_workspace(holo_handle h, size_t bytes);
 /* Per-size tile choices (0 = default): slices per batch, slices per column / row workgroup,
- * columns per workgroup, rows per workgroup. The global holo_set_tuning overrides them. */
+ * columns per workgroup, rows per workgroup. The global holo_set_tuning overrides them. The
+ * size's kernel variants (chosen by holo_autotune or shipped: elements per thread, one or two
+ * exchange buffers in the column pass, rows loaded one slice ahead in the row pass) stay. */
 int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_gen, int spg_out,
                              int cols_per_wg, int rows_per_wg);
 /* The effective choices for a size (defaults resolved) as {batch, spg_gen, spg_out, cols, rows}. */

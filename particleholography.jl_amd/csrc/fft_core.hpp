@@ -417,9 +417,12 @@ struct CPlan {
   static constexpr int T = L / E;
 };
 
-template <int R, int E, int DIR, int T, int P, int OFF, bool LAST, int PS, bool WAVE, bool DB>
+// TWR: the stage's twiddles come from registers (`twr`, R - 1 factors per butterfly q) instead of
+// the LDS table.
+template <int R, int E, int DIR, int T, int P, int OFF, bool LAST, int PS, bool WAVE, bool DB, bool TWR>
 __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const float2* __restrict__ tw,
-                                             float2* __restrict__ lds, const int es) {
+                                             float2* __restrict__ lds, const int es,
+                                             const float2* __restrict__ twr) {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
   constexpr int Q = E / R;
   int base[Q];
@@ -435,7 +438,7 @@ __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const 
       const float2* __restrict__ w = tw + OFF + k;
 #pragma unroll
       for (int m = 1; m < R; ++m) {
-        const float2 wm = w[(m - 1) * P];
+        const float2 wm = TWR ? twr[q * (R - 1) + (m - 1)] : w[(m - 1) * P];
         x[m] = DIR < 0 ? cmul(x[m], wm) : cmulconj(x[m], wm);
       }
       base[q] = j * P * R + k;
@@ -461,9 +464,40 @@ __device__ __forceinline__ void fft_stage_ct(float2 (&v)[E], const int t, const 
 // DB: exchanges alternate between `lds` and `lds + bo` (exchange s uses buffer s mod 2), so a
 // buffer is rewritten only after a barrier that follows every read of it. With an even number
 // of exchanges per line, consecutive lines of one loop keep that order too.
-template <int E, int DIR, int L, int PS, bool WAVE = false, int S = 0, bool DB = false>
+// Register twiddles of a compile-time plan: stage s >= 1 holds (E / R_s) * (R_s - 1) factors per
+// thread, stages in order. tw_regs_before(S) is the offset of stage S's block.
+template <int L, int E>
+constexpr int tw_regs_before(int S) {
+  constexpr CPlanData d = CPlan<L, E>::d;
+  int o = 0;
+  for (int s = 1; s < S && s < d.ns; ++s) o += (E / d.R[s]) * (d.R[s] - 1);
+  return o;
+}
+template <int L, int E>
+constexpr int kTwRegs = tw_regs_before<L, E>(kMaxStages) > 0 ? tw_regs_before<L, E>(kMaxStages) : 1;
+
+// Loads thread t's register twiddles from a plan table `tw` (global or LDS).
+template <int E, int L, int N>
+__device__ __forceinline__ void load_tw_regs(float2 (&twr)[N], const int t, const float2* __restrict__ tw) {
+  static_assert(N == kTwRegs<L, E>, "register twiddle array of the plan's size");
+  constexpr CPlanData d = CPlan<L, E>::d;
+  constexpr int T = CPlan<L, E>::T;
+  int o = 0;
+#pragma unroll
+  for (int s = 1; s < d.ns; ++s) {
+#pragma unroll
+    for (int q = 0; q < E / d.R[s]; ++q) {
+      const int k = (t + T * q) % d.P[s];
+#pragma unroll
+      for (int m = 1; m < d.R[s]; ++m) twr[o++] = tw[d.off[s] + k + (m - 1) * d.P[s]];
+    }
+  }
+}
+
+template <int E, int DIR, int L, int PS, bool WAVE = false, int S = 0, bool DB = false, bool TWR = false>
 __device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__ lds, const int es,
-                                            const int t, const float2* __restrict__ tw, const int bo = 0) {
+                                            const int t, const float2* __restrict__ tw, const int bo = 0,
+                                            const float2* __restrict__ twr = nullptr) {
   constexpr CPlanData d = CPlan<L, E>::d;
   if constexpr (S < d.ns) {
     constexpr int T = CPlan<L, E>::T;
@@ -473,8 +507,9 @@ __device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__
       for (int e = 0; e < E; ++e) v[e] = rd[lds_pad<PS>(t + T * e) * es];
     }
     float2* __restrict__ wr = DB && (S & 1) ? lds + bo : lds;
-    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns, PS, WAVE, DB>(v, t, tw, wr, es);
-    fft_line_ct<E, DIR, L, PS, WAVE, S + 1, DB>(v, lds, es, t, tw, bo);
+    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns, PS, WAVE, DB, TWR>(
+        v, t, tw, wr, es, TWR ? twr + tw_regs_before<L, E>(S) : nullptr);
+    fft_line_ct<E, DIR, L, PS, WAVE, S + 1, DB, TWR>(v, lds, es, t, tw, bo, twr);
   }
 }
 
@@ -505,17 +540,25 @@ __device__ __forceinline__ void fft_row(float2 (&v)[E], float2* __restrict__ lin
                                         const LinePlan& pl, const float2* __restrict__ tw) {
   fft_run<E, DIR, L, kRowPad, kWaveRow<E, L>>(v, line, 1, t, pl, tw);
 }
+// A row line with the twiddles in registers (`twr`, from load_tw_regs; compile-time plans only).
+template <int E, int DIR, int L>
+__device__ __forceinline__ void fft_row_twr(float2 (&v)[E], float2* __restrict__ line, const int t,
+                                            const float2* __restrict__ twr) {
+  static_assert(L != 0, "register twiddles need a compile-time plan");
+  fft_line_ct<E, DIR, L, kRowPad, kWaveRow<E, L>, 0, false, true>(v, line, 1, t, nullptr, 0, twr);
+}
 template <int E, int DIR, int L>
 __device__ __forceinline__ void fft_col(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,
                                         const LinePlan& pl, const float2* __restrict__ tw) {
   fft_run<E, DIR, L, kColPad>(v, col, W, t, pl, tw);
 }
-// A column line with two exchange buffers, the second `bo` elements after the first.
+// A column line with two exchange buffers, the second `bo` elements after the first, and the
+// twiddles in registers (`twr`, from load_tw_regs).
 template <int E, int DIR, int L>
 __device__ __forceinline__ void fft_col_db(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,
-                                           const float2* __restrict__ tw, const int bo) {
+                                           const float2* __restrict__ twr, const int bo) {
   static_assert(kColDB<L, E>, "double-buffered columns need a compile-time plan with an even number of exchanges");
-  fft_line_ct<E, DIR, L, kColPad, false, 0, true>(v, col, W, t, tw, bo);
+  fft_line_ct<E, DIR, L, kColPad, false, 0, true, true>(v, col, W, t, nullptr, bo, twr);
 }
 
 // Threads per line: a constant when the plan is compile-time.

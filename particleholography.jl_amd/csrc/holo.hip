@@ -59,7 +59,7 @@ struct Tuning {
   int batch = 0, spg_gen = 0, spg_out = 0, cols = 0, rows = 0;
   int erow = 0, ecol = 0;  // elements per thread of row / column passes (0: the length's default)
   int col_db = 0;          // pass A exchange buffers: 1 two, -1 one, 0 two where occupancy allows
-  int row_pf = 0;          // pass B rows loaded one slice ahead: 1 yes, -1 / 0 no
+  int row_pf = 0;          // pass B rows loaded one slice ahead: 1 yes, 2 yes with register twiddles, -1 / 0 no
 };
 
 // E = elements per thread; the set below is what the kernels are instantiated for.
@@ -570,8 +570,9 @@ int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan
 int launch_rows_proj(holo_context* h, const Geom& g, int Ex, const JobTable& jt, int spg, int groups,
                      hipStream_t st) {
   const size_t plane = static_cast<size_t>(g.nx) * g.ny;
-  const bool pf = (h->row_pf == 2 || (h->row_pf == 1 && eff_tuning(h, g.nx, g.ny).row_pf > 0)) &&
-                  plane * sizeof(float2) <= 0xffffffffull;
+  const int tpf = eff_tuning(h, g.nx, g.ny).row_pf;
+  const bool pf = (h->row_pf == 2 || (h->row_pf == 1 && tpf > 0)) && plane * sizeof(float2) <= 0xffffffffull;
+  const bool twr = pf && h->row_pf == 1 && tpf == 2;
   const size_t rlds = row_lds(g);
   const float sc2 = proj_scale(g.nx, g.ny);
   dim3 grid = row_grid(g, groups);
@@ -580,6 +581,11 @@ int launch_rows_proj(holo_context* h, const Geom& g, int Ex, const JobTable& jt,
     constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
     if constexpr (L != 0) {
+      if (twr) {
+        allow_lds(k_recon_rows_proj<E, L, true, true>, rlds);
+        hipLaunchKernelGGL((k_recon_rows_proj<E, L, true, true>), grid, row_block(g), rlds, st, jt, plane, spg, sc2, g);
+        return;
+      }
       if (pf) {
         allow_lds(k_recon_rows_proj<E, L, true>, rlds);
         hipLaunchKernelGGL((k_recon_rows_proj<E, L, true>), grid, row_block(g), rlds, st, jt, plane, spg, sc2, g);
@@ -1009,15 +1015,16 @@ int holo_create(holo_handle* out, int device) {
   // from a direct sweep: no useful round fits the 256 MB Infinity Cache at that size, so long
   // rounds (64 slices, 2 GB) amortise each launch's tail (53.8k slices/s against 48.9k with 16),
   // and its pass A keeps one exchange buffer (two: 3.88 ms against 3.76 per 200 slices). Pass B
-  // loads rows one slice ahead at 1024^2 (3.34 ms against 3.46 per 1000 slices) and 1280x1024
-  // (2.86 against 3.17 per 500); at 512^2, 768^2 and 2048^2 it was 0.2-1.4% slower.
+  // loads rows one slice ahead at 1024^2 (3.34 ms against 3.46 per 1000 slices; with the
+  // twiddles in registers 3.30) and 1280x1024 (2.86 against 3.17 per 500; its register twiddles
+  // would spill); at 512^2, 768^2 and 2048^2 it was 0.2-1.4% slower.
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
       int nx, ny, batch, spg_gen, spg_out, cols, rows, erow = 0, ecol = 0, col_db = 0, row_pf = 0;
     };
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
-                                       {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 0, 1},
+                                       {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 0, 2},
                                        {768, 768, 56, 16, 4, 4, 2},
                                        {1280, 1024, 25, 8, 8, 4, 4, 0, 0, 0, 1},
                                        {2048, 2048, 64, 16, 16, 4, 2, 0, 0, -1}};
@@ -1794,7 +1801,11 @@ int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_g
   if (!h) return fail(HOLO_ERR_ARG, "handle is NULL");
   if (cols_per_wg > 0 && (cols_per_wg & (cols_per_wg - 1)) != 0)
     return fail(HOLO_ERR_ARG, "cols_per_wg must be a power of two");
+  // the five tile fields are replaced; the size's kernel variants (elements per thread, exchange
+  // buffers, row prefetch) stay
   Tuning t;
+  auto it = h->size_tune.find({nx, ny});
+  if (it != h->size_tune.end()) t = it->second;
   t.batch = batch;
   t.spg_gen = spg_gen;
   t.spg_out = spg_out;
@@ -1982,10 +1993,10 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
           if (v < ba) { ba = v; ta = c; }
         }
     // pass B (rows, forms the projection): rows per workgroup, slices per workgroup and rows
-    // loaded one slice ahead or not
+    // loaded one slice ahead or not (with the twiddles in registers or in LDS)
     for (int rows : {1, 2, 4, 8})
       for (int so : {4, 8, 16})
-        for (int pf : {-1, 1}) {
+        for (int pf : {-1, 1, 2}) {
           Tuning c = ta;
           c.rows = rows;
           c.spg_out = std::min(so, t.batch);

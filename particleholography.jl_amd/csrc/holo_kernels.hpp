@@ -409,9 +409,16 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
   const unsigned pbytes = static_cast<unsigned>(static_cast<size_t>(g.nx) * g.ny * sizeof(float2));
   const unsigned rowb = static_cast<unsigned>(T * g.nx * sizeof(float2));
   float2 f[E];
+  // DB: the twiddles live in registers for the whole slice loop (the column exchanges are the
+  // kernel's LDS traffic)
+  float2 twr[DB ? kTwRegs<L, E> : 1];
   if constexpr (HMODE == 0) {
     col_load_buf(f, plane_rsrc(Fsrc, pbytes), off * 8u, rowb);
-    stage_twiddles(c.tw, g.twy, g.py.ntw);
+    if constexpr (DB) {
+      load_tw_regs<E, L>(twr, c.t, g.twy);
+    } else {
+      stage_twiddles(c.tw, g.twy, g.py.ntw);
+    }
   } else {
     col_load(f, Fsrc, g.nx, T, off);
   }
@@ -452,7 +459,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
         for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
       }
       if constexpr (DB) {
-        fft_col_db<E, +1, L>(v, c.col, g.W, c.t, c.tw, g.W * lds_padded_len(g.ny, kColPad));
+        fft_col_db<E, +1, L>(v, c.col, g.W, c.t, twr, g.W * lds_padded_len(g.ny, kColPad));
       } else {
         fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
       }
@@ -524,16 +531,19 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, size_t n, unsigned int 
 // PF: the next slice's row is loaded while the current one is transformed (two row buffers
 // used alternately, two slices per loop iteration), so a wave waits for memory once per
 // workgroup instead of once per slice.
-template <int E, int L, bool PF = false>
+template <int E, int L, bool PF = false, bool TWR = false>
 __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gstride, int spg,
                                                          float sc2, Geom g) {
+  static_assert(!TWR || L != 0, "register twiddles need a compile-time plan");
   extern __shared__ float2 lds[];
   const JobDesc& jd = jt.d[blockIdx.z];
   const int nbatch = jd.nbatch;
   if (static_cast<int>(blockIdx.y) * spg >= nbatch) return;
   const float2* __restrict__ G = jd.G;
   float* __restrict__ proj = jd.proj;
-  const RowCtx c = row_ctx<E, L>(g, lds);
+  const RowCtx c = row_ctx<E, L, !TWR>(g, lds);
+  float2 twr[TWR ? kTwRegs<L, E> : 1];
+  if constexpr (TWR) load_tw_regs<E, L>(twr, c.t, g.twx);
   const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.jl) * g.nx;
   const int k0 = blockIdx.y * spg;
@@ -542,7 +552,11 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
 #pragma unroll
   for (int e = 0; e < E; ++e) mn[e] = 0x7fffffffu;
   auto fold = [&](float2 (&v)[E]) {
-    fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
+    if constexpr (TWR) {
+      fft_row_twr<E, +1, L>(v, c.line, c.t, twr);
+    } else {
+      fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) mn[e] = min(mn[e], __float_as_uint(v[e].x * v[e].x + v[e].y * v[e].y));
   };
