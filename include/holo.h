@@ -300,6 +300,11 @@ int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_g
                              int cols_per_wg, int rows_per_wg);
 /* The effective choices for a size (defaults resolved) as {batch, spg_gen, spg_out, cols, rows}. */
 int holo_get_tuning_for_size(holo_handle h, int nx, int ny, int slices, int* out5);
+/* A size's kernel variants (no counterpart in the reference): pass A's exchange buffers (1 two,
+ * -1 one, 0 two where they leave as many workgroups resident per CU) and pass B's row loads
+ * (1 one slice ahead, 2 one slice ahead with the twiddles in registers, 0 or -1 in place).
+ * set != 0 stores *col_db and *row_pf for the size; set == 0 reads the size's values. */
+int holo_size_variants(holo_handle h, int nx, int ny, int* col_db, int* row_pf, int set);
 /* Times candidate tile configurations of both reconstruction passes on the device (HIP events,
  * synchronous) and keeps the fastest for this size; the tiles in force before the call stay
  * unless the winner is at least 2% faster. out5 (may be NULL) receives the tiles in force

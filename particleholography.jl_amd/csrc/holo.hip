@@ -117,7 +117,7 @@ struct holo_context {
   int max_lds = 65536;
   std::map<int, Plan1D> plans;  // key (L << 8) | E
   int col_e = 0;                 // elements per thread of column passes (0: the length's default)
-  int col_db = 1;                // 0 (HOLO_COL_DB=0): pass A always with one exchange buffer
+  int col_db = 1;                // pass A exchange buffers: HOLO_COL_DB 0 one, 1 per size, 2 two
   int row_pf = 1;                // pass B rows one slice ahead: HOLO_ROW_PF 0 never, 1 per size, 2 always
   std::map<std::tuple<const void*, int, size_t>, int> occ;  // resident workgroups per CU, by kernel
   void* ws = nullptr;
@@ -777,8 +777,8 @@ int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp,
     // two exchange buffers when the plan allows it, the tile's LDS fits twice and the size's
     // tuning asks for them (by default: when they leave as many workgroups resident per CU)
     if constexpr (kColDB<L, E>) {
-      bool db = h->col_db && tn.col_db >= 0 && lds2 <= static_cast<size_t>(h->max_lds);
-      if (db && tn.col_db == 0) {
+      bool db = (h->col_db == 2 || (h->col_db == 1 && tn.col_db >= 0)) && lds2 <= static_cast<size_t>(h->max_lds);
+      if (db && h->col_db == 1 && tn.col_db == 0) {
         allow_lds(k_recon_cols<E, L, 0, true>, lds2);
         allow_lds(k_recon_cols<E, L, 0>, lds);
         const int bs = static_cast<int>(col_block(g).x);
@@ -1017,7 +1017,9 @@ int holo_create(holo_handle* out, int device) {
   // and its pass A keeps one exchange buffer (two: 3.88 ms against 3.76 per 200 slices). Pass B
   // loads rows one slice ahead at 1024^2 (3.34 ms against 3.46 per 1000 slices; with the
   // twiddles in registers 3.30) and 1280x1024 (2.86 against 3.17 per 500; its register twiddles
-  // would spill); at 512^2, 768^2 and 2048^2 it was 0.2-1.4% slower.
+  // would spill); at 512^2, 768^2 and 2048^2 it was 0.2-1.4% slower. 1280x1024 re-tuned with
+  // both variants available: 8-column tiles, 16-slice groups, two exchange buffers (2.72 ms
+  // against 2.92 per 500 slices).
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
@@ -1026,7 +1028,7 @@ int holo_create(holo_handle* out, int device) {
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
                                        {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 0, 2},
                                        {768, 768, 56, 16, 4, 4, 2},
-                                       {1280, 1024, 25, 8, 8, 4, 4, 0, 0, 0, 1},
+                                       {1280, 1024, 25, 16, 16, 8, 4, 0, 0, 1, 1},
                                        {2048, 2048, 64, 16, 16, 4, 2, 0, 0, -1}};
     for (const Shipped& e : kShipped) {
       Tuning t;
@@ -1812,6 +1814,22 @@ int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_g
   t.cols = cols_per_wg;
   t.rows = rows_per_wg;
   h->size_tune[{nx, ny}] = t;
+  return HOLO_OK;
+}
+
+int holo_size_variants(holo_handle h, int nx, int ny, int* col_db, int* row_pf, int set) {
+  if (!h || !col_db || !row_pf) return fail(HOLO_ERR_ARG, "NULL argument");
+  if (set) {
+    if (*col_db < -1 || *col_db > 1) return fail(HOLO_ERR_ARG, "col_db must be -1, 0 or 1");
+    if (*row_pf < -1 || *row_pf > 2) return fail(HOLO_ERR_ARG, "row_pf must be -1, 0, 1 or 2");
+    Tuning& t = h->size_tune[{nx, ny}];
+    t.col_db = *col_db;
+    t.row_pf = *row_pf;
+    return HOLO_OK;
+  }
+  const Tuning t = eff_tuning(h, nx, ny);
+  *col_db = t.col_db;
+  *row_pf = t.row_pf;
   return HOLO_OK;
 }
 

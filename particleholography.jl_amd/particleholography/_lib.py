@@ -89,6 +89,7 @@ SIGNATURES = {
     "holo_reserve_workspace": [_vp, ctypes.c_size_t],
     "holo_workspace_size": [_vp, ctypes.POINTER(ctypes.c_size_t)],
     "holo_get_tuning_for_size": [_vp, _i, _i, _i, ctypes.POINTER(ctypes.c_int)],
+    "holo_size_variants": [_vp, _i, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _i],
     "holo_autotune": [_vp, _vp, _i, _i, _d, _d, ctypes.POINTER(ctypes.c_int)],
     "holo_set_profiling": [_vp, _i],
     "holo_set_gs_graphs": [_vp, _i],

@@ -76,6 +76,16 @@ def _device_key(device: int) -> str:
 
 
 FIELDS = ("batch", "spg_gen", "spg_out", "cols_per_wg", "rows_per_wg")
+VARIANTS = ("col_db", "row_pf")
+
+
+def _variants(nx: int, ny: int, device: int, set_to: Optional[dict] = None) -> dict:
+    cd, rp = ctypes.c_int(0), ctypes.c_int(0)
+    if set_to is not None:
+        cd.value, rp.value = int(set_to.get("col_db", 0)), int(set_to.get("row_pf", 0))
+    check(_lib.load().holo_size_variants(_lib.handle(device).ptr, nx, ny, ctypes.byref(cd), ctypes.byref(rp),
+                                         1 if set_to is not None else 0))
+    return {"col_db": cd.value, "row_pf": rp.value}
 
 
 def autotune(nx: int, ny: int, wavlen: float = 0.6328, dx: float = 10.0, device: int = 0,
@@ -85,6 +95,7 @@ def autotune(nx: int, ny: int, wavlen: float = 0.6328, dx: float = 10.0, device:
     stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
     check(_lib.load().holo_autotune(_lib.handle(device).ptr, stream, nx, ny, float(wavlen), float(dx), out))
     choice = dict(zip(FIELDS, list(out)))
+    choice.update(_variants(nx, ny, device))
     if save:
         path = _cache_path()
         data = json.loads(path.read_text()) if path.exists() else {}
@@ -95,14 +106,20 @@ def autotune(nx: int, ny: int, wavlen: float = 0.6328, dx: float = 10.0, device:
 
 
 def set_tuning_for_size(nx: int, ny: int, choice: dict, device: int = 0) -> None:
+    """Sets a size's tiles; its kernel variants too when ``choice`` names them."""
     vals = [int(choice.get(f, 0)) for f in FIELDS]
     check(_lib.load().holo_set_tuning_for_size(_lib.handle(device).ptr, nx, ny, *vals))
+    if any(v in choice for v in VARIANTS):
+        cur = _variants(nx, ny, device)
+        _variants(nx, ny, device, {v: choice.get(v, cur[v]) for v in VARIANTS})
 
 
 def get_tuning_for_size(nx: int, ny: int, slices: int = 0, device: int = 0) -> dict:
     out = (ctypes.c_int * 5)()
     check(_lib.load().holo_get_tuning_for_size(_lib.handle(device).ptr, nx, ny, int(slices), out))
-    return dict(zip(FIELDS, list(out)))
+    choice = dict(zip(FIELDS, list(out)))
+    choice.update(_variants(nx, ny, device))
+    return choice
 
 
 def load_tuning(device: int = 0) -> int:
