@@ -417,6 +417,133 @@ struct CPlan {
   static constexpr int T = L / E;
 };
 
+// ------------------------------------------------------------------------------------------
+// Exchanges by cross-lane swaps (gfx950 v_permlane32_swap_b32 / v_permlane16_swap_b32).
+// v_permlane32_swap(A, B) swaps lanes 32-63 of A with lanes 0-31 of B; v_permlane16_swap(A, B)
+// swaps the odd 16-lane rows of A with the even rows of B. For registers A = v[r] and
+// B = v[r | 1 << a] that is a transpose of lane bit 5 (resp. 4) with register-index bit a: after
+// it, register r' at lane l holds what register r' with bit a := l5 held at lane l with
+// l5 := bit a of r'. In a line owned by one wavefront (T = 64: lane = t; T = 32: lane bit 5 is
+// the line), an exchange whose sources differ from their destinations only in lane bits 4 and 5
+// is such a transpose followed by a renaming of registers: no LDS traffic and no ordering. For
+// 1024 = 16*16*4 and 512 = 16*16*2 at 16 elements per thread, the exchange in front of the last
+// stage is of this kind (the first one moves lane bits 0-3 and stays in LDS).
+// ------------------------------------------------------------------------------------------
+constexpr int kMaxLaneSwapE = 32;
+struct LaneSwap {
+  bool ok;
+  int rb5, rb4;                // register-index bit transposed with lane bit 5 / 4 (-1: none)
+  int sigma[kMaxLaneSwapE];    // after the swaps, v[e] takes register sigma[e]
+};
+
+// Which (thread, register) holds element n after a stage of radix R and span P with Q = E/R
+// butterflies per thread: the stage writes output m of butterfly b = t + T*q (k = b mod P,
+// j = b / P) to element j*P*R + k + m*P from register q + Q*m.
+constexpr void stage_source(int n, int R, int P, int T, int Q, int& tp, int& reg) {
+  const int k = n % P, m = (n / P) % R, j = n / (P * R);
+  const int b = j * P + k;
+  tp = b % T;
+  reg = b / T + Q * m;
+}
+
+// The swap form of the exchange into stage S (S >= 1) of the compile-time plan for (L, E), if
+// there is one; found and verified for every (thread, element) at compile time.
+constexpr LaneSwap make_lane_swap(int L, int E, int S) {
+  LaneSwap best{};
+  best.ok = false;
+  const CPlanData d = make_cplan(L, E);
+  const int T = L / E;
+  if (!d.complete || S < 1 || S >= d.ns || E > kMaxLaneSwapE || (E & (E - 1)) != 0) return best;
+  if (T != 64 && T != 32) return best;
+  const int R = d.R[S - 1], P = d.P[S - 1], Q = E / R;
+  int ebits = 0;
+  while ((1 << ebits) < E) ++ebits;
+  for (int a = -1; a < ebits; ++a) {
+    for (int c = -1; c < ebits; ++c) {
+      if ((a < 0 && c < 0) || (a >= 0 && a == c) || (T == 32 && a >= 0)) continue;
+      const int moved = (a >= 0 ? 0x20 : 0) | (c >= 0 ? 0x10 : 0);
+      const int rmask = (a >= 0 ? 1 << a : 0) | (c >= 0 ? 1 << c : 0);
+      LaneSwap s{};
+      s.ok = true;
+      s.rb5 = a;
+      s.rb4 = c;
+      int used = 0;
+      for (int e = 0; e < E && s.ok; ++e) {
+        int t0 = 0, r0 = 0;
+        stage_source(T * e, R, P, T, Q, t0, r0);
+        if ((t0 & ~moved) != 0 || (r0 & rmask) != 0) { s.ok = false; break; }
+        int sg = r0;
+        if (a >= 0) sg |= ((t0 >> 5) & 1) << a;
+        if (c >= 0) sg |= ((t0 >> 4) & 1) << c;
+        if ((used >> sg) & 1) { s.ok = false; break; }
+        used |= 1 << sg;
+        s.sigma[e] = sg;
+        for (int t = 0; t < T && s.ok; ++t) {
+          int tp = 0, r = 0;
+          stage_source(t + T * e, R, P, T, Q, tp, r);
+          int want_t = t & ~moved, want_r = sg & ~rmask;
+          if (a >= 0) { want_t |= ((sg >> a) & 1) << 5; want_r |= ((t >> 5) & 1) << a; }
+          if (c >= 0) { want_t |= ((sg >> c) & 1) << 4; want_r |= ((t >> 4) & 1) << c; }
+          if (tp != want_t || r != want_r) s.ok = false;
+        }
+      }
+      if (s.ok) return s;
+    }
+  }
+  return best;
+}
+
+template <int L, int E, int S>
+struct LaneSwapPlan {
+  static constexpr LaneSwap s = make_lane_swap(L, E, S);
+};
+
+__device__ __forceinline__ void swap_lanes32(float& a, float& b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap_lanes16(float& a, float& b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+
+// The exchange into stage S by lane swaps: on exit v[e] holds element t + T*e, as after the LDS
+// read. Every lane of the wavefront must be active.
+template <int L, int E, int S>
+__device__ __forceinline__ void lane_swap_exchange(float2 (&v)[E]) {
+  constexpr LaneSwap sw = LaneSwapPlan<L, E, S>::s;
+  static_assert(sw.ok, "exchange has no lane-swap form");
+  if constexpr (sw.rb5 >= 0) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      if ((r >> sw.rb5) & 1) continue;
+      swap_lanes32(v[r].x, v[r | (1 << sw.rb5)].x);
+      swap_lanes32(v[r].y, v[r | (1 << sw.rb5)].y);
+    }
+  }
+  if constexpr (sw.rb4 >= 0) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      if ((r >> sw.rb4) & 1) continue;
+      swap_lanes16(v[r].x, v[r | (1 << sw.rb4)].x);
+      swap_lanes16(v[r].y, v[r | (1 << sw.rb4)].y);
+    }
+  }
+  float2 u[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) u[e] = v[sw.sigma[e]];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = u[e];
+}
+
+// The exchange into stage S goes by lane swaps when the line's layout allows it (XS: a line owned
+// by one wavefront, or a column line whose thread bits 4 and 5 sit in lane bits 4 and 5) and
+// the exchange has that form.
+template <int L, int E, int S, bool XS>
+constexpr bool kSwapExchange = XS && L != 0 && LaneSwapPlan<L, E, S>::s.ok;
+
 // TWR: the stage's twiddles come from registers (`twr`, R - 1 factors per butterfly q) instead of
 // the LDS table.
 template <int R, int E, int DIR, int T, int P, int OFF, bool LAST, int PS, bool WAVE, bool DB, bool TWR>
@@ -494,22 +621,28 @@ __device__ __forceinline__ void load_tw_regs(float2 (&twr)[N], const int t, cons
   }
 }
 
-template <int E, int DIR, int L, int PS, bool WAVE = false, int S = 0, bool DB = false, bool TWR = false>
+// XS: exchanges of the swap form go by lane swaps (default: lines owned by one wavefront).
+template <int E, int DIR, int L, int PS, bool WAVE = false, int S = 0, bool DB = false, bool TWR = false,
+          bool XS = WAVE>
 __device__ __forceinline__ void fft_line_ct(float2 (&v)[E], float2* __restrict__ lds, const int es,
                                             const int t, const float2* __restrict__ tw, const int bo = 0,
                                             const float2* __restrict__ twr = nullptr) {
   constexpr CPlanData d = CPlan<L, E>::d;
   if constexpr (S < d.ns) {
     constexpr int T = CPlan<L, E>::T;
-    if constexpr (S > 0) {
+    if constexpr (S > 0 && kSwapExchange<L, E, S, XS>) {
+      lane_swap_exchange<L, E, S>(v);
+    } else if constexpr (S > 0) {
       const float2* __restrict__ rd = DB && ((S - 1) & 1) ? lds + bo : lds;
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = rd[lds_pad<PS>(t + T * e) * es];
     }
     float2* __restrict__ wr = DB && (S & 1) ? lds + bo : lds;
-    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], S + 1 == d.ns, PS, WAVE, DB, TWR>(
+    // the stage skips its LDS writes when it is the last or the next exchange goes by lane swaps
+    constexpr bool kNoWrite = S + 1 == d.ns || kSwapExchange<L, E, S + 1, XS>;
+    fft_stage_ct<d.R[S], E, DIR, T, d.P[S], d.off[S], kNoWrite, PS, WAVE, DB, TWR>(
         v, t, tw, wr, es, TWR ? twr + tw_regs_before<L, E>(S) : nullptr);
-    fft_line_ct<E, DIR, L, PS, WAVE, S + 1, DB, TWR>(v, lds, es, t, tw, bo, twr);
+    fft_line_ct<E, DIR, L, PS, WAVE, S + 1, DB, TWR, XS>(v, lds, es, t, tw, bo, twr);
   }
 }
 
@@ -558,7 +691,33 @@ template <int E, int DIR, int L>
 __device__ __forceinline__ void fft_col_db(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,
                                            const float2* __restrict__ twr, const int bo) {
   static_assert(kColDB<L, E>, "double-buffered columns need a compile-time plan with an even number of exchanges");
-  fft_line_ct<E, DIR, L, kColPad, false, 0, true, true>(v, col, W, t, nullptr, bo, twr);
+  fft_line_ct<E, DIR, L, kColPad, false, 0, true, true, false>(v, col, W, t, nullptr, bo, twr);
+}
+
+// Column lines whose last exchange goes by lane swaps and whose single LDS exchange uses the
+// buffer `col` chosen by the caller: compile-time plans with one line per 64 threads, one LDS
+// exchange and the swap form for the rest (1024 = 16*16*4 at 16 elements per thread). The tile's
+// threads must be laid out by col_ctx's swizzled mapping (thread bits 4 and 5 in lane bits 4 and
+// 5). The exchange is not preceded by a barrier: a caller that alternates two buffers between
+// consecutive transforms has every read of a buffer ordered before its next write by the
+// barrier of the transform in between.
+template <int L, int E>
+constexpr bool colxs_ok() {
+  if constexpr (L == 0) {
+    return false;
+  } else {
+    constexpr CPlanData d = CPlan<L, E>::d;
+    if (!d.complete || L / E != 64 || d.ns != 3) return false;
+    return !LaneSwapPlan<L, E, 1>::s.ok && LaneSwapPlan<L, E, 2>::s.ok;
+  }
+}
+template <int L, int E>
+constexpr bool kColXS = colxs_ok<L, E>();
+template <int E, int DIR, int L, bool TWR>
+__device__ __forceinline__ void fft_col_xs(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,
+                                           const float2* __restrict__ tw, const float2* __restrict__ twr) {
+  static_assert(kColXS<L, E>, "no lane-swap column form for this plan");
+  fft_line_ct<E, DIR, L, kColPad, false, 0, true, TWR, true>(v, col, W, t, tw, 0, twr);
 }
 
 // Threads per line: a constant when the plan is compile-time.

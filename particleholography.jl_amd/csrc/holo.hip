@@ -58,7 +58,7 @@ struct Plan1D {
 struct Tuning {
   int batch = 0, spg_gen = 0, spg_out = 0, cols = 0, rows = 0;
   int erow = 0, ecol = 0;  // elements per thread of row / column passes (0: the length's default)
-  int col_db = 0;          // pass A exchange buffers: 1 two, -1 one, 0 two where occupancy allows
+  int col_db = 0;          // pass A exchanges: 1 two buffers, -1 one, 0 two where occupancy allows; 2 / 3 lane swaps + buffers by slice (twiddles in LDS / registers)
   int row_pf = 0;          // pass B rows loaded one slice ahead: 1 yes, 2 yes with register twiddles, -1 / 0 no
 };
 
@@ -757,6 +757,16 @@ int reconstruct(holo_context* h, hipStream_t st, int nx, int ny, const void* src
   return HOLO_OK;
 }
 
+// Whether the column pass over lines of length ny (at the length's default elements per thread)
+// has the lane-swap variants (col_db 2 and 3).
+bool col_xs_form(int ny) {
+  bool xs = false;
+  dispatch_EL(choose_E(ny), ny, [&](auto ec, auto lc) {
+    xs = kColXS<decltype(lc)::value, decltype(ec)::value>;
+  });
+  return xs;
+}
+
 // Pass A of a generated-H round: the column pass that generates H_k, over (column tiles, slice
 // groups, holograms).
 int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp, double wavlen, double dx,
@@ -774,19 +784,34 @@ int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp,
     constexpr int L = decltype(lc)::value;
     dim3 grid = col_grid(g, (maxnb + rp.spg_gen - 1) / rp.spg_gen);
     grid.z = jt.n;
+    // the last exchange by lane swaps and the other in a buffer alternating by slice, where the
+    // size's tuning asks for it (2: twiddles from LDS, 3: in registers) and the plan has that form
+    if constexpr (kColXS<L, E>) {
+      if (h->col_db == 1 && (tn.col_db == 2 || tn.col_db == 3) && lds2 <= static_cast<size_t>(h->max_lds)) {
+        if (tn.col_db == 2) {
+          allow_lds(k_recon_cols<E, L, 0, 2>, lds2);
+          hipLaunchKernelGGL((k_recon_cols<E, L, 0, 2>), grid, col_block(g), lds2, st, ra, jt, g);
+        } else {
+          const size_t lx = lds2 - g.py.ntw * sizeof(float2);  // no table: twiddles in registers
+          allow_lds(k_recon_cols<E, L, 0, 3>, lx);
+          hipLaunchKernelGGL((k_recon_cols<E, L, 0, 3>), grid, col_block(g), lx, st, ra, jt, g);
+        }
+        return;
+      }
+    }
     // two exchange buffers when the plan allows it, the tile's LDS fits twice and the size's
     // tuning asks for them (by default: when they leave as many workgroups resident per CU)
     if constexpr (kColDB<L, E>) {
       bool db = (h->col_db == 2 || (h->col_db == 1 && tn.col_db >= 0)) && lds2 <= static_cast<size_t>(h->max_lds);
       if (db && h->col_db == 1 && tn.col_db == 0) {
-        allow_lds(k_recon_cols<E, L, 0, true>, lds2);
+        allow_lds(k_recon_cols<E, L, 0, 1>, lds2);
         allow_lds(k_recon_cols<E, L, 0>, lds);
         const int bs = static_cast<int>(col_block(g).x);
-        db = resident(h, k_recon_cols<E, L, 0, true>, bs, lds2) >= resident(h, k_recon_cols<E, L, 0>, bs, lds);
+        db = resident(h, k_recon_cols<E, L, 0, 1>, bs, lds2) >= resident(h, k_recon_cols<E, L, 0>, bs, lds);
       }
       if (db) {
-        allow_lds(k_recon_cols<E, L, 0, true>, lds2);
-        hipLaunchKernelGGL((k_recon_cols<E, L, 0, true>), grid, col_block(g), lds2, st, ra, jt, g);
+        allow_lds(k_recon_cols<E, L, 0, 1>, lds2);
+        hipLaunchKernelGGL((k_recon_cols<E, L, 0, 1>), grid, col_block(g), lds2, st, ra, jt, g);
         return;
       }
     }
@@ -1019,16 +1044,19 @@ int holo_create(holo_handle* out, int device) {
   // twiddles in registers 3.30) and 1280x1024 (2.86 against 3.17 per 500; its register twiddles
   // would spill); at 512^2, 768^2 and 2048^2 it was 0.2-1.4% slower. 1280x1024 re-tuned with
   // both variants available: 8-column tiles, 16-slice groups, two exchange buffers (2.72 ms
-  // against 2.92 per 500 slices).
+  // against 2.92 per 500 slices). Sides whose columns are 1024 tall run pass A with its last
+  // exchange by lane swaps and one barrier per slice (col_db 3): 1024^2 3.23 -> 3.11-3.14 ms per
+  // 1000 slices (pass A 56.7 -> 52.7 us per 32-slice launch), 1280x1024 2.52 -> 2.39 ms per 500;
+  // with the twiddles from LDS instead (col_db 2) 3.19 and 2.45.
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
       int nx, ny, batch, spg_gen, spg_out, cols, rows, erow = 0, ecol = 0, col_db = 0, row_pf = 0;
     };
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
-                                       {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 0, 2},
+                                       {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 3, 2},
                                        {768, 768, 56, 16, 4, 4, 2},
-                                       {1280, 1024, 25, 16, 16, 8, 4, 0, 0, 1, 1},
+                                       {1280, 1024, 25, 16, 16, 8, 4, 0, 0, 3, 1},
                                        {2048, 2048, 64, 16, 16, 4, 2, 0, 0, -1}};
     for (const Shipped& e : kShipped) {
       Tuning t;
@@ -1820,7 +1848,7 @@ int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_g
 int holo_size_variants(holo_handle h, int nx, int ny, int* col_db, int* row_pf, int set) {
   if (!h || !col_db || !row_pf) return fail(HOLO_ERR_ARG, "NULL argument");
   if (set) {
-    if (*col_db < -1 || *col_db > 1) return fail(HOLO_ERR_ARG, "col_db must be -1, 0 or 1");
+    if (*col_db < -1 || *col_db > 3) return fail(HOLO_ERR_ARG, "col_db must be -1, 0, 1, 2 or 3");
     if (*row_pf < -1 || *row_pf > 2) return fail(HOLO_ERR_ARG, "row_pf must be -1, 0, 1 or 2");
     Tuning& t = h->size_tune[{nx, ny}];
     t.col_db = *col_db;
@@ -2000,7 +2028,8 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     // two exchange buffers
     for (int W : {2, 4, 8, 16})
       for (int sg : {4, 8, 16})
-        for (int db : {-1, 1}) {
+        for (int db : {-1, 1, 2, 3}) {
+          if (db >= 2 && !col_xs_form(ny)) continue;
           Tuning c = t;
           c.cols = W;
           c.spg_gen = std::min(sg, t.batch);

@@ -272,11 +272,22 @@ __device__ __forceinline__ void col_store_buf(const float2 (&v)[E], __amdgpu_buf
 }
 
 // NBUF exchange buffers of the tile's W columns, then the twiddle table.
-template <int E, int L, bool STAGE = true, int NBUF = 1>
+// Thread layout: column w = thread & (W - 1) (W adjacent columns per 64-bit access segment). By
+// default the line's thread index is t = thread >> log2(W). SWZ (64-thread lines, W <= 16): the
+// lane bits above log2(W) carry the same bits of t and the wave index the low log2(W) bits, so
+// t's bits 4 and 5 sit in lane bits 4 and 5 and an exchange of the lane-swap form stays in
+// registers (fft_col_xs). A wave then covers 64 / W rows t0, t0 + W, ... of W columns each, as
+// many W*8-byte segments per access as the default layout's 64 / W consecutive rows.
+template <int E, int L, bool STAGE = true, int NBUF = 1, bool SWZ = false>
 __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   ColCtx c;
   c.w = threadIdx.x & (g.W - 1);
-  c.t = threadIdx.x >> g.wshift;
+  if constexpr (SWZ) {
+    static_assert(L != 0 && L / E == 64, "the swizzled layout is for 64-thread lines");
+    c.t = (threadIdx.x & 63 & ~(g.W - 1)) | (threadIdx.x >> 6);
+  } else {
+    c.t = threadIdx.x >> g.wshift;
+  }
   c.i = xcd_tile(blockIdx.x, gridDim.x) * g.W + c.w;
   c.col = lds + c.w;
   float2* tw = lds + NBUF * g.W * lds_padded_len(g.ny, kColPad);
@@ -391,17 +402,25 @@ __device__ __forceinline__ size_t centred_index(int i, int j, int nx, int ny) {
   return static_cast<size_t>(jc) * nx + ic;
 }
 
-// DB (HMODE 0 only): the slice loop's column FFTs alternate between two exchange buffers,
-// two workgroup barriers per slice instead of four.
-template <int E, int L, int HMODE, bool DB = false>
+// XB, the slice loop's column exchanges (HMODE 0 only for XB > 0):
+//   0: one LDS buffer, two workgroup barriers per exchange (four per slice at 1024 = 16*16*4);
+//   1: two buffers alternating by exchange, one barrier per exchange, twiddles in registers;
+//   2: the last exchange by lane swaps (swizzled thread layout, fft_col_xs) and the other in a
+//      buffer alternating by slice: one barrier per slice; twiddles from the LDS table;
+//   3: as 2 with the twiddles in registers.
+template <int E, int L, int HMODE, int XB = 0>
 __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom g) {
-  static_assert(!DB || HMODE == 0, "double buffering is for the generated-H slice loop");
+  static_assert(XB == 0 || HMODE == 0, "the exchange variants are for the generated-H slice loop");
+  static_assert(XB < 2 || kColXS<L, E>, "lane-swap columns need a plan of that form");
+  constexpr bool DB = XB == 1;
+  constexpr bool XS = XB >= 2;
+  constexpr bool TWR = XB == 1 || XB == 3;
   extern __shared__ float2 lds[];
   // HMODE 0: hologram blockIdx.z of the table; workgroups past its slice count leave at once
   const JobDesc& jd = jt.d[HMODE == 0 ? blockIdx.z : 0];
   if (HMODE == 0 && static_cast<int>(blockIdx.y) * a.spg >= jd.nbatch) return;
   const float2* __restrict__ Fsrc = HMODE == 0 ? jd.F : a.F;
-  const ColCtx c = col_ctx<E, L, false, DB ? 2 : 1>(g, lds);
+  const ColCtx c = col_ctx<E, L, false, XB >= 1 ? 2 : 1, XS>(g, lds);
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
@@ -409,12 +428,12 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
   const unsigned pbytes = static_cast<unsigned>(static_cast<size_t>(g.nx) * g.ny * sizeof(float2));
   const unsigned rowb = static_cast<unsigned>(T * g.nx * sizeof(float2));
   float2 f[E];
-  // DB: the twiddles live in registers for the whole slice loop (the column exchanges are the
+  // TWR: the twiddles live in registers for the whole slice loop (the column exchanges are the
   // kernel's LDS traffic)
-  float2 twr[DB ? kTwRegs<L, E> : 1];
+  float2 twr[TWR ? kTwRegs<L, E> : 1];
   if constexpr (HMODE == 0) {
     col_load_buf(f, plane_rsrc(Fsrc, pbytes), off * 8u, rowb);
-    if constexpr (DB) {
+    if constexpr (TWR) {
       load_tw_regs<E, L>(twr, c.t, g.twy);
     } else {
       stage_twiddles(c.tw, g.twy, g.py.ntw);
@@ -458,7 +477,10 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
 #pragma unroll
         for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
       }
-      if constexpr (DB) {
+      if constexpr (XS) {
+        fft_col_xs<E, +1, L, TWR>(v, c.col + ((k & 1) ? g.W * lds_padded_len(g.ny, kColPad) : 0), g.W, c.t,
+                                  c.tw, twr);
+      } else if constexpr (DB) {
         fft_col_db<E, +1, L>(v, c.col, g.W, c.t, twr, g.W * lds_padded_len(g.ny, kColPad));
       } else {
         fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);

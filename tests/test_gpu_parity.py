@@ -148,6 +148,51 @@ def test_projection_batching_is_invisible(gpu, batch, spgc, spgr):
     assert asm.rel_l2(got, base) < 1e-6
 
 
+@pytest.mark.parametrize("nx,ny", [(1024, 1024), (1280, 1024)])
+def test_pass_a_exchange_variants_bitwise_equal(gpu, nx, ny):
+    """Pass A's exchange variants move the same values through different routes: one LDS buffer;
+    two alternating by exchange; the last exchange by lane swaps (v_permlane16/32_swap, swizzled
+    thread layout) with the other in two buffers alternating by slice, twiddles from LDS or from
+    registers. The projection is bitwise the same for each, at tile widths 2-8, over a slice
+    count that leaves a short last group (odd slice counts per group use both buffers first),
+    and within the bar of the FP64 oracle."""
+    import ctypes
+
+    from particleholography import _lib, batch
+
+    slices = 37
+    h = _holo(nx, 21, ny)
+    ht = _t(h, gpu)
+    lib = _lib.load()
+    proj = torch.empty((ny, nx), dtype=torch.float32, device=gpu)
+    f = _lib.holo_tf(None, -2e4, LAM, DX, 1)
+    d = _lib.holo_tf(None, -333.0, LAM, DX, 1)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    hd = _lib.handle(0)
+    v0 = batch._variants(nx, ny, 0)
+    outs = {}
+    try:
+        for cd in (-1, 1, 2, 3):
+            batch._variants(nx, ny, 0, {"col_db": cd, "row_pf": v0["row_pf"]})
+            for w in (2, 4, 8):
+                hd.set_tuning(0, 0, 0, w, 0)
+                _lib.check(lib.holo_reconst_xyprojection_holo(hd.ptr, st, nx, ny, ctypes.c_void_p(ht.data_ptr()),
+                                                               ctypes.byref(f), ctypes.byref(d), slices,
+                                                               ctypes.c_void_p(proj.data_ptr())))
+                outs[(cd, w)] = proj.cpu().numpy()
+    finally:
+        hd.set_tuning()
+        batch._variants(nx, ny, 0, v0)
+    base = outs[(-1, 8)]
+    for k, v in outs.items():
+        assert np.array_equal(v, base), f"col_db {k[0]}, {k[1]} columns per workgroup"
+    want = asm.reconst_xyprojection(asm.gabor_wavefront(h), slices, "exact", z_front=-2e4, z_dz=-333.0,
+                                    wavlen=LAM, dx=DX)
+    err = asm.rel_l2(base, want)
+    print(f"pass-A exchange variants {nx}x{ny} x {slices}: bitwise equal, {err:.2e} rel L2 vs FP64 oracle")
+    assert err < 1e-4
+
+
 @pytest.mark.parametrize("nx,ny", [(80, 64), (64, 96), (160, 48)])
 def test_projection_non_square(gpu, nx, ny):
     from particleholography import _lib
