@@ -300,9 +300,12 @@ int holo_set_tuning_for_size(holo_handle h, int nx, int ny, int batch, int spg_g
                              int cols_per_wg, int rows_per_wg);
 /* The effective choices for a size (defaults resolved) as {batch, spg_gen, spg_out, cols, rows}. */
 int holo_get_tuning_for_size(holo_handle h, int nx, int ny, int slices, int* out5);
-/* A size's kernel variants (no counterpart in the reference): pass A's exchange buffers (1 two,
- * -1 one, 0 two where they leave as many workgroups resident per CU) and pass B's row loads
- * (1 one slice ahead, 2 one slice ahead with the twiddles in registers, 0 or -1 in place).
+/* A size's kernel variants (no counterpart in the reference): pass A's exchanges (1 two buffers,
+ * -1 one, 0 two where they leave as many workgroups resident per CU; for sides whose columns are
+ * 1024 tall, 2 and 3: the last exchange by cross-lane swaps and one LDS exchange per slice in
+ * buffers alternating by slice, with the twiddles in LDS (2) or in registers (3)) and pass B's row loads
+ * (1 one slice ahead, 2 one slice ahead with the twiddles in registers, 3 as 2 with every row
+ * exchange through LDS instead of the last one by cross-lane swaps, 0 or -1 in place).
  * set != 0 stores *col_db and *row_pf for the size; set == 0 reads the size's values. */
 int holo_size_variants(holo_handle h, int nx, int ny, int* col_db, int* row_pf, int set);
 /* Times candidate tile configurations of both reconstruction passes on the device (HIP events,

@@ -674,11 +674,13 @@ __device__ __forceinline__ void fft_row(float2 (&v)[E], float2* __restrict__ lin
   fft_run<E, DIR, L, kRowPad, kWaveRow<E, L>>(v, line, 1, t, pl, tw);
 }
 // A row line with the twiddles in registers (`twr`, from load_tw_regs; compile-time plans only).
-template <int E, int DIR, int L>
+// XS false keeps every exchange in LDS.
+template <int E, int DIR, int L, bool XS = true>
 __device__ __forceinline__ void fft_row_twr(float2 (&v)[E], float2* __restrict__ line, const int t,
                                             const float2* __restrict__ twr) {
   static_assert(L != 0, "register twiddles need a compile-time plan");
-  fft_line_ct<E, DIR, L, kRowPad, kWaveRow<E, L>, 0, false, true>(v, line, 1, t, nullptr, 0, twr);
+  fft_line_ct<E, DIR, L, kRowPad, kWaveRow<E, L>, 0, false, true, XS && kWaveRow<E, L>>(v, line, 1, t, nullptr, 0,
+                                                                                        twr);
 }
 template <int E, int DIR, int L>
 __device__ __forceinline__ void fft_col(float2 (&v)[E], float2* __restrict__ col, const int W, const int t,

@@ -59,7 +59,7 @@ struct Tuning {
   int batch = 0, spg_gen = 0, spg_out = 0, cols = 0, rows = 0;
   int erow = 0, ecol = 0;  // elements per thread of row / column passes (0: the length's default)
   int col_db = 0;          // pass A exchanges: 1 two buffers, -1 one, 0 two where occupancy allows; 2 / 3 lane swaps + buffers by slice (twiddles in LDS / registers)
-  int row_pf = 0;          // pass B rows loaded one slice ahead: 1 yes, 2 yes with register twiddles, -1 / 0 no
+  int row_pf = 0;          // pass B rows loaded one slice ahead: 1 yes, 2 yes with register twiddles, 3 as 2 with every exchange in LDS, -1 / 0 no
 };
 
 // E = elements per thread; the set below is what the kernels are instantiated for.
@@ -119,6 +119,9 @@ struct holo_context {
   int col_e = 0;                 // elements per thread of column passes (0: the length's default)
   int col_db = 1;                // pass A exchange buffers: HOLO_COL_DB 0 one, 1 per size, 2 two
   int row_pf = 1;                // pass B rows one slice ahead: HOLO_ROW_PF 0 never, 1 per size, 2 always
+  int gs_xs = 1;                 // single-plane column passes over 1024-tall columns: HOLO_GS_XS 0 the
+                                 // plane tiles' 8 elements per thread, 1 / 2 16 elements with lane-swap
+                                 // exchanges and one / two LDS buffers
   std::map<std::tuple<const void*, int, size_t>, int> occ;  // resident workgroups per CU, by kernel
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -331,11 +334,15 @@ Tuning eff_tuning(const holo_context* h, int nx, int ny) {
 // per thread (more waves, one more LDS exchange) and narrower tiles. Measured on MI355X (E rows / columns, iterations/s): 1024^2 E 4/8 on 4-column and
 // 1-row tiles 20.9k (16.2k at the reconstruction's E 16/16, 20.3k at E 8/8); 512^2 E 4/8, 4 x 1
 // 42.0k; 2048^2 E 8/16, 4 x 1 8.7k. Global tuning (holo_set_tuning) still sets the tile sizes.
+// 1024-tall columns take 16 elements per thread with the lane-swap exchanges (HOLO_GS_XS 1, the
+// default): each transform's last exchange in registers, three barriers per column pass instead
+// of twelve at 8 elements; batched GS 1024^2 x 16 pairs 53.1k -> 56.6k pair-iterations/s (32
+// pairs 52.2k -> 53.8k), single pairs and cu_asm_prop! unchanged.
 Tuning plane_tiles(const holo_context* h, int nx, int ny) {
   Tuning t;
   if (nx == 1024 && ny == 1024) {
     t.erow = 4;
-    t.ecol = 8;
+    t.ecol = h->gs_xs > 0 ? 0 : 8;
     t.cols = 4;
     t.rows = 1;
   } else if (nx == 512 && ny == 512) {
@@ -572,7 +579,8 @@ int launch_rows_proj(holo_context* h, const Geom& g, int Ex, const JobTable& jt,
   const size_t plane = static_cast<size_t>(g.nx) * g.ny;
   const int tpf = eff_tuning(h, g.nx, g.ny).row_pf;
   const bool pf = (h->row_pf == 2 || (h->row_pf == 1 && tpf > 0)) && plane * sizeof(float2) <= 0xffffffffull;
-  const bool twr = pf && h->row_pf == 1 && tpf == 2;
+  const bool twr = pf && h->row_pf == 1 && tpf >= 2;
+  const bool xs = tpf != 3;  // 3: register twiddles with every exchange through LDS
   const size_t rlds = row_lds(g);
   const float sc2 = proj_scale(g.nx, g.ny);
   dim3 grid = row_grid(g, groups);
@@ -581,6 +589,12 @@ int launch_rows_proj(holo_context* h, const Geom& g, int Ex, const JobTable& jt,
     constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
     if constexpr (L != 0) {
+      if (twr && !xs) {
+        allow_lds(k_recon_rows_proj<E, L, true, true, false>, rlds);
+        hipLaunchKernelGGL((k_recon_rows_proj<E, L, true, true, false>), grid, row_block(g), rlds, st, jt, plane,
+                           spg, sc2, g);
+        return;
+      }
       if (twr) {
         allow_lds(k_recon_rows_proj<E, L, true, true>, rlds);
         hipLaunchKernelGGL((k_recon_rows_proj<E, L, true, true>), grid, row_block(g), rlds, st, jt, plane, spg, sc2, g);
@@ -846,8 +860,10 @@ int launch_round(holo_context* h, const Geom& g, int Ex, int Ey, const ReconPlan
 // One propagation column pass over a single plane: in -> FFT_col -> .* H(t) -> IFFT_col -> out
 // (H generated per element, or read from the centred array; cu_asm_prop!).
 // Column pass of a propagation (transfer `t`) or, when `lpf` is given, of a low-pass filter.
+// xb: the column exchanges of k_prop_cols where the plan has the lane-swap form (0 all in LDS,
+// 1 lane swaps with one LDS buffer, 2 with two buffers where they fit `max_lds`).
 int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, float2* out,
-                     hipStream_t st, const float* lpf = nullptr) {
+                     hipStream_t st, const float* lpf = nullptr, int xb = 0, int max_lds = 65536) {
   PropArgs pa{};
   pa.in = in;
   pa.out = out;
@@ -862,33 +878,61 @@ int launch_prop_cols(const Geom& g, int Ey, const holo_tf* t, const float2* in, 
     pa.lpf = lpf;
   }
   GsTable gt{};
-  const size_t lds = col_lds(g);
   bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
-    auto launch = [&](auto kern) {
+    auto launch = [&](auto kern, int nbuf) {
+      const size_t lds = col_lds(g, nbuf);
       allow_lds(kern, lds);
       hipLaunchKernelGGL(kern, col_grid(g), col_block(g), lds, st, pa, gt, g);
     };
-    if (mode == 0) launch(k_prop_cols<E, L, 0>);
-    else if (mode == 1) launch(k_prop_cols<E, L, 1>);
-    else launch(k_prop_cols<E, L, 3>);
+    if constexpr (kColXS<L, E>) {
+      if (xb == 1 || (xb == 2 && col_lds(g, 2) > static_cast<size_t>(max_lds))) {
+        if (mode == 0) launch(k_prop_cols<E, L, 0, 1>, 1);
+        else if (mode == 1) launch(k_prop_cols<E, L, 1, 1>, 1);
+        else launch(k_prop_cols<E, L, 3, 1>, 1);
+        return;
+      }
+      if (xb == 2) {
+        if (mode == 0) launch(k_prop_cols<E, L, 0, 2>, 2);
+        else if (mode == 1) launch(k_prop_cols<E, L, 1, 2>, 2);
+        else launch(k_prop_cols<E, L, 3, 2>, 2);
+        return;
+      }
+    }
+    if (mode == 0) launch(k_prop_cols<E, L, 0>, 1);
+    else if (mode == 1) launch(k_prop_cols<E, L, 1>, 1);
+    else launch(k_prop_cols<E, L, 3>, 1);
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
   return check_launch();
 }
 
 // GS column pass for every pair of the table (natural-order transfer arrays).
-int launch_prop_cols_table(const Geom& g, int Ey, const GsTable& gt, hipStream_t st) {
+int launch_prop_cols_table(const Geom& g, int Ey, const GsTable& gt, hipStream_t st, int xb = 0,
+                           int max_lds = 65536) {
   PropArgs pa{};
-  const size_t lds = col_lds(g);
   bool ok = dispatch_EL(Ey, g.ny, [&](auto ec, auto lc) {
     constexpr int E = decltype(ec)::value;
     constexpr int L = decltype(lc)::value;
-    allow_lds(k_prop_cols<E, L, 2>, lds);
     dim3 grid = col_grid(g);
     grid.z = gt.n;
-    hipLaunchKernelGGL((k_prop_cols<E, L, 2>), grid, col_block(g), lds, st, pa, gt, g);
+    auto launch = [&](auto kern, int nbuf) {
+      const size_t lds = col_lds(g, nbuf);
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, grid, col_block(g), lds, st, pa, gt, g);
+    };
+    if constexpr (kColXS<L, E>) {
+      if (xb == 1 || (xb == 2 && col_lds(g, 2) > static_cast<size_t>(max_lds))) {
+        launch(k_prop_cols<E, L, 2, 1>, 1);
+        return;
+      }
+      if (xb == 2) {
+        launch(k_prop_cols<E, L, 2, 2>, 2);
+        return;
+      }
+    }
+    launch(k_prop_cols<E, L, 2>, 1);
   });
   if (!ok) return fail(HOLO_ERR_SIZE, "no column kernel for this size");
   return check_launch();
@@ -942,7 +986,7 @@ int gs_round(holo_context* h, const Geom& g, int Ex, int Ey, const GsPair* pairs
   const GsTable* rows[2] = {&r2, &r1};
   for (int half = 0; half < 2; ++half) {
     hipEvent_t pc = prof_begin(h, st);
-    HOLO_TRY(launch_prop_cols_table(g, Ey, *cols[half], st));
+    HOLO_TRY(launch_prop_cols_table(g, Ey, *cols[half], st, h->gs_xs, h->max_lds));
     prof_end(h, st, pc, HOLO_PASS_PROP_COLS, cols[half]->n);
     hipEvent_t pr = prof_begin(h, st);
     HOLO_TRY(launch_gs_rows_table(g, Ex, *rows[half], sc, st));
@@ -1035,6 +1079,7 @@ int holo_create(holo_handle* out, int device) {
   h->col_e = static_cast<int>(env_size("HOLO_COL_E", 0));
   h->col_db = static_cast<int>(env_size("HOLO_COL_DB", 1));
   h->row_pf = static_cast<int>(env_size("HOLO_ROW_PF", 1));
+  h->gs_xs = static_cast<int>(env_size("HOLO_GS_XS", 1));
   // Per-size tiles chosen by holo_autotune on an MI355X and confirmed on single-hologram
   // reconstructions (768^2 and 1280x1024 +37%, 512^2 +5% over the generic rule). 2048^2 comes
   // from a direct sweep: no useful round fits the 256 MB Infinity Cache at that size, so long
@@ -1047,14 +1092,15 @@ int holo_create(holo_handle* out, int device) {
   // against 2.92 per 500 slices). Sides whose columns are 1024 tall run pass A with its last
   // exchange by lane swaps and one barrier per slice (col_db 3): 1024^2 3.23 -> 3.11-3.14 ms per
   // 1000 slices (pass A 56.7 -> 52.7 us per 32-slice launch), 1280x1024 2.52 -> 2.39 ms per 500;
-  // with the twiddles from LDS instead (col_db 2) 3.19 and 2.45.
+  // with the twiddles from LDS instead (col_db 2) 3.19 and 2.45. Pass B at 1024^2 keeps both row
+  // exchanges in LDS (row_pf 3: 46.2 against 46.9 us per launch with the last one by lane swaps).
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
       int nx, ny, batch, spg_gen, spg_out, cols, rows, erow = 0, ecol = 0, col_db = 0, row_pf = 0;
     };
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
-                                       {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 3, 2},
+                                       {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 3, 3},
                                        {768, 768, 56, 16, 4, 4, 2},
                                        {1280, 1024, 25, 16, 16, 8, 4, 0, 0, 3, 1},
                                        {2048, 2048, 64, 16, 16, 4, 2, 0, 0, -1}};
@@ -1192,7 +1238,7 @@ int holo_asm_prop(holo_handle h, void* stream, int nx, int ny, void* out, const 
   hipStream_t st = static_cast<hipStream_t>(stream);
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
   HOLO_TRY(launch_rows_fft(g, Ex, -1, 0, in, 0, B, 0, 1.0f, 1, st));
-  HOLO_TRY(launch_prop_cols(g, Ey, tf, B, B, st));
+  HOLO_TRY(launch_prop_cols(g, Ey, tf, B, B, st, nullptr, h->gs_xs, h->max_lds));
   return launch_rows_fft(g, Ex, +1, 0, B, 0, static_cast<float2*>(out), 0, sc, 1, st);
 }
 
@@ -1232,7 +1278,7 @@ int holo_apply_low_pass_filter(holo_handle h, void* stream, int nx, int ny, void
   hipStream_t st = static_cast<hipStream_t>(stream);
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
   HOLO_TRY(launch_rows_fft(g, Ex, -1, 0, in, 0, B, 0, 1.0f, 1, st));
-  HOLO_TRY(launch_prop_cols(g, Ey, nullptr, B, B, st, lpf));
+  HOLO_TRY(launch_prop_cols(g, Ey, nullptr, B, B, st, lpf, h->gs_xs, h->max_lds));
   return launch_rows_fft(g, Ex, +1, 0, B, 0, static_cast<float2*>(out), 0, sc, 1, st);
 }
 
@@ -1849,7 +1895,7 @@ int holo_size_variants(holo_handle h, int nx, int ny, int* col_db, int* row_pf, 
   if (!h || !col_db || !row_pf) return fail(HOLO_ERR_ARG, "NULL argument");
   if (set) {
     if (*col_db < -1 || *col_db > 3) return fail(HOLO_ERR_ARG, "col_db must be -1, 0, 1, 2 or 3");
-    if (*row_pf < -1 || *row_pf > 2) return fail(HOLO_ERR_ARG, "row_pf must be -1, 0, 1 or 2");
+    if (*row_pf < -1 || *row_pf > 3) return fail(HOLO_ERR_ARG, "row_pf must be -1, 0, 1, 2 or 3");
     Tuning& t = h->size_tune[{nx, ny}];
     t.col_db = *col_db;
     t.row_pf = *row_pf;
@@ -2043,7 +2089,7 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     // loaded one slice ahead or not (with the twiddles in registers or in LDS)
     for (int rows : {1, 2, 4, 8})
       for (int so : {4, 8, 16})
-        for (int pf : {-1, 1, 2}) {
+        for (int pf : {-1, 1, 2, 3}) {
           Tuning c = ta;
           c.rows = rows;
           c.spg_out = std::min(so, t.batch);

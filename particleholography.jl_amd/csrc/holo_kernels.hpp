@@ -552,8 +552,9 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, size_t n, unsigned int 
 // equals the per-slice scaled form bit for bit.
 // PF: the next slice's row is loaded while the current one is transformed (two row buffers
 // used alternately, two slices per loop iteration), so a wave waits for memory once per
-// workgroup instead of once per slice.
-template <int E, int L, bool PF = false, bool TWR = false>
+// workgroup instead of once per slice. TWR: the twiddles in registers; XS false (with TWR):
+// every exchange through LDS instead of the last one by lane swaps.
+template <int E, int L, bool PF = false, bool TWR = false, bool XS = true>
 __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gstride, int spg,
                                                          float sc2, Geom g) {
   static_assert(!TWR || L != 0, "register twiddles need a compile-time plan");
@@ -575,7 +576,7 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
   for (int e = 0; e < E; ++e) mn[e] = 0x7fffffffu;
   auto fold = [&](float2 (&v)[E]) {
     if constexpr (TWR) {
-      fft_row_twr<E, +1, L>(v, c.line, c.t, twr);
+      fft_row_twr<E, +1, L, XS>(v, c.line, c.t, twr);
     } else {
       fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
     }
@@ -760,13 +761,17 @@ struct PropArgs {
   const float* lpf;    // HMODE 3: centred real filter
 };
 
-template <int E, int L, int HMODE>
+// XB: 0 every exchange in LDS; 1 and 2 (columns of the lane-swap form, swizzled thread layout)
+// each transform's last exchange by lane swaps, its other one in LDS: one buffer shared by the
+// two transforms (three barriers) or one buffer each (two barriers).
+template <int E, int L, int HMODE, int XB = 0>
 __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom g) {
+  static_assert(XB == 0 || kColXS<L, E>, "lane-swap columns need a plan of that form");
   extern __shared__ float2 lds[];
   const GsDesc& pd = gt.d[HMODE == 2 ? blockIdx.z : 0];
   const float2* __restrict__ src = HMODE == 2 ? pd.B : a.in;
   float2* __restrict__ dst = HMODE == 2 ? pd.B : a.out;
-  const ColCtx c = col_ctx<E, L, false>(g, lds);
+  const ColCtx c = col_ctx<E, L, false, XB == 2 ? 2 : 1, (XB > 0)>(g, lds);
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
@@ -784,7 +789,11 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
     if constexpr (HMODE == 3) lv[e] = a.lpf[centred_index(ci, c.t + T * e, g.nx, g.ny)];
   }
   stage_twiddles(c.tw, g.twy, g.py.ntw);
-  fft_col<E, -1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+  if constexpr (XB > 0) {
+    fft_col_xs<E, -1, L, false>(v, c.col, g.W, c.t, c.tw, nullptr);
+  } else {
+    fft_col<E, -1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+  }
   if constexpr (HMODE == 0) {
     const double ax = axis_a_s(ci, g.nx, axis_scale(g.nx, a.gp.wavlen, a.gp.dx));
     const double sy = axis_scale(g.ny, a.gp.wavlen, a.gp.dx);
@@ -804,7 +813,14 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = make_float2(lv[e] * v[e].x, lv[e] * v[e].y);
   }
-  fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+  if constexpr (XB == 1) {
+    __syncthreads();  // every thread has read the first transform's exchange
+    fft_col_xs<E, +1, L, false>(v, c.col, g.W, c.t, c.tw, nullptr);
+  } else if constexpr (XB == 2) {
+    fft_col_xs<E, +1, L, false>(v, c.col + g.W * lds_padded_len(g.ny, kColPad), g.W, c.t, c.tw, nullptr);
+  } else {
+    fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
+  }
   if (ok) {
 #pragma unroll
     for (int e = 0; e < E; ++e) dst[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
