@@ -192,17 +192,21 @@ def main():
     alg_bytes = pass_a_bytes(tune["spg_gen"]) * N * N * d["slices"] / launches
     achieved = alg_bytes / avg_s / 1e9
     traffic, dram = None, None
+    # the pass-A instantiation in use: <E, L, generated H, exchange variant>; the size's col_db
+    # 1-3 name the variant, -1 and 0 (two buffers where occupancy allows) leave it to the library
+    xb = tune["col_db"] if tune["col_db"] in (1, 2, 3) else None
+    kname = f"k_recon_cols<16, {N}, 0, {xb}>" if xb is not None else f"k_recon_cols<16, {N}, 0, *>"
     pmc_file = ROOT / "profiles" / "pmc_traffic_kernels.json"
     if pmc_file.exists():  # HBM bytes per launch from the committed rocprofv3 PMC runs
         pmc = json.loads(pmc_file.read_text())
         for kn, v in pmc["kernels"].items():
-            if "k_recon_cols<" in kn and f", {N}, 0>" in kn:
+            if (xb is not None and kname in kn) or (xb is None and f"k_recon_cols<16, {N}, 0," in kn):
                 traffic = int(v["fetch_bytes_mean"] + v["write_bytes_mean"])
                 dram = v.get("dram_read_requests_mean")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": pmc_file.name if traffic else None,
-                "kernel": "k_recon_cols<16, 1024, 0> (pass A)", "avg_launch_ms": round(avg_s * 1e3, 4),
+                "kernel": f"{kname} (pass A)", "avg_launch_ms": round(avg_s * 1e3, 4),
                 "bytes_per_launch": alg_bytes,
                 "bytes_model": f"8 B/px/slice written + the 8 B/px spectrum read once per {tune['spg_gen']}-slice group",
                 "counter_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
