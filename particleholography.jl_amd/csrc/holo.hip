@@ -802,13 +802,18 @@ int launch_gen_cols(holo_context* h, const Geom& g, int Ey, const ReconPlan& rp,
     // size's tuning asks for it (2: twiddles from LDS, 3: in registers) and the plan has that form
     if constexpr (kColXS<L, E>) {
       if (h->col_db == 1 && (tn.col_db == 2 || tn.col_db == 3) && lds2 <= static_cast<size_t>(h->max_lds)) {
+        // 8-column tiles (the shipped width) with the width compiled in
+        auto launch = [&](auto kern, size_t bytes) {
+          allow_lds(kern, bytes);
+          hipLaunchKernelGGL(kern, grid, col_block(g), bytes, st, ra, jt, g);
+        };
+        const size_t lx = lds2 - g.py.ntw * sizeof(float2);  // no table: twiddles in registers
         if (tn.col_db == 2) {
-          allow_lds(k_recon_cols<E, L, 0, 2>, lds2);
-          hipLaunchKernelGGL((k_recon_cols<E, L, 0, 2>), grid, col_block(g), lds2, st, ra, jt, g);
+          if (g.W == 8) launch(k_recon_cols<E, L, 0, 2, 8>, lds2);
+          else launch(k_recon_cols<E, L, 0, 2>, lds2);
         } else {
-          const size_t lx = lds2 - g.py.ntw * sizeof(float2);  // no table: twiddles in registers
-          allow_lds(k_recon_cols<E, L, 0, 3>, lx);
-          hipLaunchKernelGGL((k_recon_cols<E, L, 0, 3>), grid, col_block(g), lx, st, ra, jt, g);
+          if (g.W == 8) launch(k_recon_cols<E, L, 0, 3, 8>, lx);
+          else launch(k_recon_cols<E, L, 0, 3>, lx);
         }
         return;
       }

@@ -278,19 +278,21 @@ __device__ __forceinline__ void col_store_buf(const float2 (&v)[E], __amdgpu_buf
 // t's bits 4 and 5 sit in lane bits 4 and 5 and an exchange of the lane-swap form stays in
 // registers (fft_col_xs). A wave then covers 64 / W rows t0, t0 + W, ... of W columns each, as
 // many W*8-byte segments per access as the default layout's 64 / W consecutive rows.
-template <int E, int L, bool STAGE = true, int NBUF = 1, bool SWZ = false>
+// WC: the tile width as a compile-time constant (0: g.W at run time).
+template <int E, int L, bool STAGE = true, int NBUF = 1, bool SWZ = false, int WC = 0>
 __device__ __forceinline__ ColCtx col_ctx(const Geom& g, float2* lds) {
   ColCtx c;
-  c.w = threadIdx.x & (g.W - 1);
+  const int W = WC > 0 ? WC : g.W;
+  c.w = threadIdx.x & (W - 1);
   if constexpr (SWZ) {
     static_assert(L != 0 && L / E == 64, "the swizzled layout is for 64-thread lines");
-    c.t = (threadIdx.x & 63 & ~(g.W - 1)) | (threadIdx.x >> 6);
+    c.t = (threadIdx.x & 63 & ~(W - 1)) | (threadIdx.x >> 6);
   } else {
-    c.t = threadIdx.x >> g.wshift;
+    c.t = WC > 0 ? static_cast<int>(threadIdx.x) / WC : static_cast<int>(threadIdx.x >> g.wshift);
   }
-  c.i = xcd_tile(blockIdx.x, gridDim.x) * g.W + c.w;
+  c.i = xcd_tile(blockIdx.x, gridDim.x) * W + c.w;
   c.col = lds + c.w;
-  float2* tw = lds + NBUF * g.W * lds_padded_len(g.ny, kColPad);
+  float2* tw = lds + NBUF * W * lds_padded_len(L != 0 ? L : g.ny, kColPad);
   if (STAGE) stage_twiddles(tw, g.twy, g.py.ntw);
   c.tw = tw;
   return c;
@@ -408,7 +410,9 @@ __device__ __forceinline__ size_t centred_index(int i, int j, int nx, int ny) {
 //   2: the last exchange by lane swaps (swizzled thread layout, fft_col_xs) and the other in a
 //      buffer alternating by slice: one barrier per slice; twiddles from the LDS table;
 //   3: as 2 with the twiddles in registers.
-template <int E, int L, int HMODE, int XB = 0>
+// WC: the tile width as a compile-time constant (0: g.W), which folds the LDS addresses of the
+// exchange into immediate offsets.
+template <int E, int L, int HMODE, int XB = 0, int WC = 0>
 __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom g) {
   static_assert(XB == 0 || HMODE == 0, "the exchange variants are for the generated-H slice loop");
   static_assert(XB < 2 || kColXS<L, E>, "lane-swap columns need a plan of that form");
@@ -420,7 +424,9 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
   const JobDesc& jd = jt.d[HMODE == 0 ? blockIdx.z : 0];
   if (HMODE == 0 && static_cast<int>(blockIdx.y) * a.spg >= jd.nbatch) return;
   const float2* __restrict__ Fsrc = HMODE == 0 ? jd.F : a.F;
-  const ColCtx c = col_ctx<E, L, false, XB >= 1 ? 2 : 1, XS>(g, lds);
+  static_assert(WC == 0 || XS, "compile-time tile widths are for the lane-swap variants");
+  const ColCtx c = col_ctx<E, L, false, XB >= 1 ? 2 : 1, XS, WC>(g, lds);
+  const int W = WC > 0 ? WC : g.W;
   const int T = line_threads<E, L>(g.py);
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
@@ -469,6 +475,21 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
         }
       }
     }
+    if constexpr (XS) {
+      // consecutive slices alternate between the two exchange buffers
+      const int bo = W * lds_padded_len(L, kColPad);
+      for (int k = k0; k < k1; ++k) {
+        float2 v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = f[e];
+        if (k + 1 < k1) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
+        }
+        fft_col_xs<E, +1, L, TWR>(v, c.col + ((k & 1) ? bo : 0), W, c.t, c.tw, twr);
+        col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? off * 8u : pbytes, rowb);
+      }
+    } else {
     for (int k = k0; k < k1; ++k) {
       float2 v[E];
 #pragma unroll
@@ -477,16 +498,14 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
 #pragma unroll
         for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
       }
-      if constexpr (XS) {
-        fft_col_xs<E, +1, L, TWR>(v, c.col + ((k & 1) ? g.W * lds_padded_len(g.ny, kColPad) : 0), g.W, c.t,
-                                  c.tw, twr);
-      } else if constexpr (DB) {
+      if constexpr (DB) {
         fft_col_db<E, +1, L>(v, c.col, g.W, c.t, twr, g.W * lds_padded_len(g.ny, kColPad));
       } else {
         fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
       }
       // columns past the plane (the last tile) get an offset past the buffer: dropped stores
       col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? off * 8u : pbytes, rowb);
+    }
     }
   } else {
     float2 hdz[E];

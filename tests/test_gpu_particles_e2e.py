@@ -5,10 +5,12 @@ saved from that test (tests/golden/particles.json).
 
 The comparison is loose by construction: the file comes from cuFFT and CUDA on another machine,
 and linking thresholded voxels along z is sensitive to single-LSB differences of the volume. On
-this data, every saved particle has a detected particle within 1.5 px in x and y. The depths
-of 20 of the 22 agree within 5 slices; the two others are neighbourhoods whose depth profile has
-two peaks. The saved depths lie on average about one slice (1.16 +- 0.22) above the smoothed
-profile peaks computed here, which follow the current code of holofunc.jl:184-200."""
+this data, every saved particle has a detected particle within 1.5 px in x and y. Depth here
+minus saved depth, per saved particle, is 0 (3 particles), +1 (12), +2 (4), +5, and for the two
+neighbourhoods whose depth profile has two peaks +93 and +154 slices. The shift is a whole
+slice: the saved depths are what a volume whose first slice lies at z0 + dz gives (the slice-step
+transfer applied before slice 1 is stored), while holofunc.jl:189-197, followed here, stores
+slice 1 at z0 (pinned by the volume parity tests). The test fixes that one-slice shift."""
 import json
 from pathlib import Path
 
@@ -47,6 +49,8 @@ def test_particles_of_the_reference_holograms(gpu):
         j = int(np.argmin(dxy))
         assert dxy[j] <= 1.5, f"no particle near saved particle {p}"
         dz_found.append(ours[j, 2] - p[2])
-    dz_found = np.abs(np.array(dz_found))
-    assert (dz_found <= 5).sum() >= 20, dz_found
-    assert np.median(dz_found) <= 2, dz_found
+    print("depth here minus saved depth, per saved particle (slices):", sorted(float(d) for d in dz_found))
+    dz_found = np.array(dz_found)
+    assert np.median(dz_found) == 1.0, dz_found  # the whole-slice shift of the saved file
+    assert (np.abs(dz_found - 1.0) <= 1.0).sum() >= 18, dz_found
+    assert (np.abs(dz_found) <= 5).sum() >= 20, dz_found
