@@ -9,8 +9,9 @@ The serving loop (one host thread):
 * requests arrive over time (their ``arrival`` offsets) into a FIFO;
 * admission: the oldest waiting requests are admitted while their device-memory footprint
   (inputs, outputs, retrieved wavefronts) fits the budget left by the in-flight work and the
-  slots' reserved workspaces, up to ``max_batch`` per launch and ``max_group_work``
-  algorithmic bytes per group (32 GB, about 5 ms of work: a group finishes when its slowest
+  slots' reserved workspaces, up to ``max_batch`` per launch and ``max_group_work`` bytes of
+  work per group (the reference-traffic measure of ``Request.work_bytes``; 32 GB, about 5 ms
+  of work: a group finishes when its slowest
   request does, so small groups keep the tail short -- measured p99 211 -> ~120 ms at 75% load
   with unchanged capacity);
 * an admitted group runs on a free slot -- a default-priority stream with its own libholo
@@ -70,9 +71,16 @@ class Request:
         return b
 
     def work_bytes(self) -> int:
-        """Algorithmic HBM bytes: 32 B/px per slice, 72 B/px per GS iteration."""
+        """The request's work measure for admission: the HBM bytes of the reference's per-slice
+        sequence, 32 B/px per slice and 72 B/px per GS iteration ("reference traffic")."""
         ny, nx = self.holo.shape
         return nx * ny * (32 * self.slices + 72 * self.gs_iters)
+
+    def fused_bytes(self, px_bytes_per_slice: float) -> float:
+        """Algorithmic HBM bytes of the fused design: ``px_bytes_per_slice`` (16 + 8/spg_gen +
+        8/spg_out for the size's tiles) per pixel and slice, 72 B/px per GS iteration."""
+        ny, nx = self.holo.shape
+        return nx * ny * (px_bytes_per_slice * self.slices + 72.0 * self.gs_iters)
 
 
 @dataclass
@@ -219,11 +227,31 @@ class Server:
         return summarize(requests, makespan, groups)
 
 
+def fused_px_bytes(sizes, device: int = 0) -> dict:
+    """Fused-design bytes per pixel and slice for each (ny, nx): the spectrum read once per
+    pass-A slice group, the column-transformed slice written and read, the projection
+    read-modify-written once per pass-B group (16 + 8/spg_gen + 8/spg_out)."""
+    from .batch import get_tuning_for_size
+
+    out = {}
+    for (ny, nx) in sizes:
+        t = get_tuning_for_size(nx, ny, 0, device)
+        out[(ny, nx)] = 16.0 + 8.0 / t["spg_gen"] + 8.0 / t["spg_out"]
+    return out
+
+
+def _traffic(requests: List[Request], makespan: float) -> dict:
+    """Reference-traffic and fused-design (algorithmic) GB/s of a served set of requests."""
+    ref = sum(r.work_bytes() for r in requests)
+    pxb = fused_px_bytes({tuple(r.holo.shape) for r in requests})
+    alg = sum(r.fused_bytes(pxb[tuple(r.holo.shape)]) for r in requests)
+    return {"algorithmic_GBs": alg / makespan / 1e9, "reference_traffic_GBs": ref / makespan / 1e9}
+
+
 def summarize(requests: List[Request], makespan: float, groups: int = 0) -> dict:
     lat = np.array([r.t_done - r.arrival for r in requests])
     slices = sum(r.slices for r in requests)
     eq1024 = sum(r.slices * r.holo.shape[0] * r.holo.shape[1] for r in requests) / (1024 * 1024)
-    work = sum(r.work_bytes() for r in requests)
     per_size = {}
     for (ny, nx) in sorted({tuple(r.holo.shape) for r in requests}):
         ls = np.array([r.t_done - r.arrival for r in requests if tuple(r.holo.shape) == (ny, nx)])
@@ -232,7 +260,7 @@ def summarize(requests: List[Request], makespan: float, groups: int = 0) -> dict
     return {"requests": len(requests), "groups": groups, "makespan_s": makespan,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "slices_per_s": slices / makespan,
-            "slices_1024eq_per_s": eq1024 / makespan, "algorithmic_GBs": work / makespan / 1e9,
+            "slices_1024eq_per_s": eq1024 / makespan, **_traffic(requests, makespan),
             "per_size": per_size}
 
 
@@ -290,9 +318,8 @@ def _single_recon(src, r):
 
 def summarize_closed(requests: List[Request], makespan: float) -> dict:
     slices = sum(r.slices for r in requests)
-    work = sum(r.work_bytes() for r in requests)
     return {"requests": len(requests), "makespan_s": makespan, "slices_per_s": slices / makespan,
-            "algorithmic_GBs": work / makespan / 1e9}
+            **_traffic(requests, makespan)}
 
 
 SIZES = [(512, 512), (768, 768), (1024, 1024), (1280, 1024), (2048, 2048)]
