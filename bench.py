@@ -14,6 +14,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -194,13 +195,17 @@ def main():
     traffic, dram = None, None
     # the pass-A instantiation in use: <E, L, generated H, exchange variant>; the size's col_db
     # 1-3 name the variant, -1 and 0 (two buffers where occupancy allows) leave it to the library
+    # (a fifth argument is the tile width when it is compiled in)
     xb = tune["col_db"] if tune["col_db"] in (1, 2, 3) else None
-    kname = f"k_recon_cols<16, {N}, 0, {xb}>" if xb is not None else f"k_recon_cols<16, {N}, 0, *>"
+    wc = ", 8" if xb in (2, 3) and tune["cols_per_wg"] == 8 else ""
+    kname = f"k_recon_cols<16, {N}, 0, {xb}{wc}>" if xb is not None else f"k_recon_cols<16, {N}, 0, *>"
+    xb_pat = str(xb) if xb is not None else r"-?\d+"
+    kpat = re.compile(rf"k_recon_cols<16, {N}, 0, {xb_pat}(, \d+)?>")
     pmc_file = ROOT / "profiles" / "pmc_traffic_kernels.json"
     if pmc_file.exists():  # HBM bytes per launch from the committed rocprofv3 PMC runs
         pmc = json.loads(pmc_file.read_text())
         for kn, v in pmc["kernels"].items():
-            if (xb is not None and kname in kn) or (xb is None and f"k_recon_cols<16, {N}, 0," in kn):
+            if kpat.search(kn):
                 traffic = int(v["fetch_bytes_mean"] + v["write_bytes_mean"])
                 dram = v.get("dram_read_requests_mean")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -235,6 +240,7 @@ def main():
         extras["ingest_1024"] = bench_ingest(ph, torch, dev)
         extras["detect_reference_holograms"] = bench_detect(ph, torch, dev)
         extras["batched_64"] = bench_batched(torch, dev, args.batch_holograms)
+        extras["hbm_practical"] = bench_hbm_practical(torch, dev)
         if args.serve_requests > 0:
             extras["serving"] = bench_serving(torch, args.serve_requests, plain)
 
@@ -519,6 +525,24 @@ def bench_serving(torch, nreq, plain_1024):
             "capacity_speedup_vs_plain": round(cap["slices_1024eq_per_s"] / plain_eq, 2),
             "load75_speedup_vs_plain": round(live["slices_1024eq_per_s"] / plain_eq, 2),
             "workspace_grew_after_warmup": grew}
+
+
+def bench_hbm_practical(torch, dev, nbytes=1 << 30, reps=10):
+    """What plain streaming kernels reach on this device (torch's fill and device copy over 1 GiB
+    of float32): the practical write and read+write ceilings next to the 8 TB/s published peak
+    that every frac above is quoted against."""
+    n = nbytes // 4
+    a = torch.empty(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    out = {}
+    for name, fn, moved in (("fill", lambda: a.fill_(2.0), nbytes), ("copy", lambda: b.copy_(a), 2 * nbytes)):
+        _, sec = timed_steps(fn, reps, 2, torch)
+        out[f"{name}_GBs"] = round(moved * reps / sec / 1e9, 1)
+    out["workload"] = "1 GiB float32: fill (write), device copy (read + write)"
+    del a, b
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_gs_batched(torch, dev, iters=100):
