@@ -192,6 +192,35 @@ def test_projection_from_volume_nan_rule():
     assert asm.xy_projection_from_vol(u8)[0, 0] == np.float32(13) / np.float32(255)
 
 
+@pytest.mark.parametrize("where", ["front", "dz", "none"])
+def test_projection_nan_rules_agree_on_reconstructed_volumes(where):
+    """The reference's projection from a volume keeps a NaN of slice 1 and ignores later ones
+    (holofunc.jl:268-282); the build takes fminf over all slices (a NaN counts only where every
+    slice is NaN). On volumes the reconstruction produces the two rules agree: a NaN anywhere in
+    the spectrum times the transfer makes the whole inverse FFT NaN, and the running product
+    (holofunc.jl:189-197) carries it into every later slice. A NaN planted in one element of the
+    front transfer makes every slice NaN; one in the slice-step transfer every slice after the
+    first."""
+    rng = np.random.default_rng(11)
+    n, slices = 16, 5
+    wf = asm.gabor_wavefront((1.0 + 0.2 * rng.standard_normal((n, n))).clip(0).astype(np.float32))
+    d = asm.transfer_sqrt_arr(n, 0.6328, 10.0)
+    h0 = asm.transfer_reference(-3e3, 0.6328, d).astype(np.complex64)
+    hdz = asm.transfer_reference(-150.0, 0.6328, d).astype(np.complex64)
+    if where == "front":
+        h0[3, 5] = np.nan
+    elif where == "dz":
+        hdz[7, 2] = np.nan
+    vol = asm.reconst_vol(wf, slices, np.float32, "reference", h_front=h0, h_dz=hdz)
+    nan_slices = [bool(np.isnan(v).all()) for v in vol]
+    assert all(np.isnan(v).all() or not np.isnan(v).any() for v in vol)  # NaN-ness is slice-wide
+    assert nan_slices == {"front": [True] * slices, "dz": [False] + [True] * (slices - 1),
+                          "none": [False] * slices}[where]
+    ref_rule = asm.xy_projection_from_vol(vol)
+    fmin_rule = np.fmin.reduce(vol, axis=0).astype(np.float32)
+    np.testing.assert_array_equal(ref_rule, fmin_rule)
+
+
 # ---------------------------------------------------------------------------------------
 # f2: cu_2d_pad known-answer test (runtests.jl:362-368)
 # ---------------------------------------------------------------------------------------
