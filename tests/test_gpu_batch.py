@@ -48,6 +48,36 @@ def test_batch_uneven_sizes_equal_single_calls(gpu):
         np.testing.assert_array_equal(out.cpu().numpy(), want.cpu().numpy())
 
 
+def test_batch_configured_mix_at_its_depths(gpu):
+    """The batched workload's shapes at its depths: every configured side, a 2048^2 hologram
+    included, with 128-1024 slices over the 1e4-2e5 um range in one call. Each result equals its
+    single call bit for bit (shipped tiles: 32-slice rounds at 1024^2, 25 at 1280x1024, 64 at
+    2048^2, so every depth crosses several rounds); the 512^2 and 768^2 results also against the
+    FP64 oracle."""
+    import particleholography as ph
+    from particleholography.batch import ReconJob, reconst_batch
+
+    cases = [(512, 512, 128), (768, 768, 300), (1024, 1024, 1024), (1280, 1024, 257), (2048, 2048, 140),
+             (512, 512, 777)]
+    jobs, holos = [], []
+    for i, (nx, ny, s) in enumerate(cases):
+        h = _holo(nx, ny, 700 + i)
+        holos.append(h)
+        jobs.append(ReconJob(torch.from_numpy(h).to(gpu), -1e4, -(2e5 - 1e4) / s, s, LAM, DX))
+    outs = [o.cpu().numpy() for o in reconst_batch(jobs)]
+    for (nx, ny, s), j, out in zip(cases, jobs, outs):
+        want = _single(ph, j.holo, j.z_front, j.z_dz, s, nx, ny).cpu().numpy()
+        np.testing.assert_array_equal(out, want, err_msg=f"{nx}x{ny} x {s}")
+    for (nx, ny, s), h, j, out in zip(cases, holos, jobs, outs):
+        if nx > 768 or s > 300:
+            continue
+        want = asm.reconst_xyprojection(asm.gabor_wavefront(h), s, "exact", z_front=j.z_front, z_dz=j.z_dz,
+                                        wavlen=LAM, dx=DX)
+        err = asm.rel_l2(out, want)
+        print(f"batched {nx}x{ny} x {s} to z = 2e5 um: rel L2 vs FP64 oracle {err:.2e}")
+        assert err < 1e-4
+
+
 def test_batch_vs_fp64_oracle_and_chunking(gpu):
     from particleholography.batch import ReconJob, reconst_batch
 
