@@ -1324,8 +1324,10 @@ int holo_make_background(holo_handle h, void* stream, int nx, int ny, int count,
     if (count <= 256) {  // 8-bit counters
       const bool vec4 = (npx % 4 == 0) && (reinterpret_cast<uintptr_t>(stack) % 4 == 0);
       const bool buf = vec4 && npx * static_cast<size_t>(count) <= 0x7fffffffull;
+      // buffer loads: 16 phases of 512 threads share a tile (256 x 1024^2: 0.067 ms against
+      // 0.071 with 8 phases, twice the waves per CU at almost the same LDS)
       if (buf)
-        hipLaunchKernelGGL(k_background_mode8<2>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
+        hipLaunchKernelGGL((k_background_mode8<2, 16>), dim3(gx), dim3(512), 0, st, npx, count, stack, out);
       else if (vec4)
         hipLaunchKernelGGL(k_background_mode8<1>, dim3(gx), dim3(kModeThreads), 0, st, npx, count, stack, out);
       else

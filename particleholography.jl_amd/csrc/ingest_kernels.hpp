@@ -140,29 +140,33 @@ __global__ __launch_bounds__(kModeThreads) void k_background_mode(size_t npx, in
 // The same mode for stacks of at most 256 images with 8-bit counters: a word holds the counters
 // of four consecutive pixels, so a 128-pixel tile needs 32 KB of LDS and four workgroups fit a
 // CU (twice the loads in flight of the 16-bit kernel). Thread (quad p = t % 32, phase
-// q = t / 32) counts images q, q+8, ... below 255, so no counter passes 255; the 256th image,
-// if any, is added in registers while phase q scans bins [32 q, 32 q + 32).
+// q = t / 32, PH phases) counts images q, q+PH, ... below 255, so no counter passes 255; the
+// 256th image, if any, is added in registers while phase q scans bins [256/PH q, 256/PH (q+1)).
 // LOAD 0: byte loads; 1 (npx % 4 == 0, stack 4-byte aligned): one uchar4 load per quad and
 // image; 2 (as 1, and the stack within 2 GiB): the same words through a buffer resource, the
-// image stride in the scalar offset and one 32-bit offset register per thread, all 32 of a
-// thread's images in flight at once (8 phases x 32 = the whole 256-image stack); images past
-// the stack read past the buffer's range and return 0 (their counts are skipped).
+// image stride in the scalar offset and one 32-bit offset register per thread, all of a
+// thread's images in flight at once (PH phases x 256/PH = the whole 256-image stack); images
+// past the stack read past the buffer's range and return 0 (their counts are skipped).
 // The LDS histogram bounds the pixels a CU can hold, so the images in flight per pixel set the
 // bytes in flight: 16 per thread gave 64 KB per CU (0.11 ms for 256 x 1024^2).
+// PH phases share a tile (32 * PH threads per workgroup): 8, or 16 for twice the waves per CU
+// at almost the same LDS (each thread then counts and scans half as much).
 constexpr int kMode8Quads = 32;
-constexpr int kMode8Phases = kModeThreads / kMode8Quads;  // 8
-template <int LOAD>
-constexpr int kMode8Unroll = LOAD == 2 ? 32 : 16;
-template <int LOAD>
-__global__ __launch_bounds__(kModeThreads) void k_background_mode8(size_t npx, int count,
-                                                                    const uint8_t* __restrict__ stack,
-                                                                    double* __restrict__ out) {
+template <int LOAD, int PH>
+constexpr int kMode8Unroll = LOAD == 2 ? 256 / PH : 16;
+template <int LOAD, int PH = 8>
+__global__ __launch_bounds__(32 * PH) void k_background_mode8(size_t npx, int count,
+                                                              const uint8_t* __restrict__ stack,
+                                                              double* __restrict__ out) {
   constexpr bool VEC4 = LOAD >= 1;
-  constexpr int kUnroll = kMode8Unroll<LOAD>;
+  constexpr int kUnroll = kMode8Unroll<LOAD, PH>;
+  constexpr int kMode8Phases = PH;
+  constexpr int kThreads = 32 * PH;
+  constexpr int kBins = 256 / PH;  // bins scanned per phase
   __shared__ __align__(16) unsigned int hist[256 * kMode8Quads];
   __shared__ unsigned int best[kMode8Phases][4 * kMode8Quads];
   const int t = threadIdx.x;
-  for (int k = t; k < 256 * kMode8Quads / 4; k += kModeThreads) reinterpret_cast<uint4*>(hist)[k] = make_uint4(0u, 0u, 0u, 0u);
+  for (int k = t; k < 256 * kMode8Quads / 4; k += kThreads) reinterpret_cast<uint4*>(hist)[k] = make_uint4(0u, 0u, 0u, 0u);
   __syncthreads();
   const int p = t % kMode8Quads;
   const int q = t / kMode8Quads;
@@ -210,13 +214,13 @@ __global__ __launch_bounds__(kModeThreads) void k_background_mode8(size_t npx, i
     }
   }
   __syncthreads();
-  // Phase q scans bins [32 q, 32 q + 32) of its quad. Each count and bin form the key
+  // Phase q scans bins [kBins q, kBins (q + 1)) of its quad. Each count and bin form the key
   // count << 8 | (255 - bin): the largest key is the most frequent value, the smallest bin on
   // ties (the reference's first strict maximum). One byte permute builds the 16-bit keys of two
   // pixels from a counter word and the replicated 255 - bin, one packed 16-bit max keeps them.
   typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
   u16x2 a01 = {0, 0}, a23 = {0, 0};
-  for (int b = 32 * q; b < 32 * q + 32; ++b) {
+  for (int b = kBins * q; b < kBins * (q + 1); ++b) {
     const unsigned int w = hist[b * kMode8Quads + p];
     const unsigned int kk = (255u - static_cast<unsigned>(b)) * 0x01010101u;
     // bytes of {kk, w}: 0-3 the counters of pixels 0-3, 4-7 copies of 255 - bin
