@@ -467,7 +467,7 @@ constexpr LaneSwap make_lane_swap(int L, int E, int S) {
       s.ok = true;
       s.rb5 = a;
       s.rb4 = c;
-      int used = 0;
+      unsigned long long used = 0;  // registers already taken as sources (E <= 32)
       for (int e = 0; e < E && s.ok; ++e) {
         int t0 = 0, r0 = 0;
         stage_source(T * e, R, P, T, Q, t0, r0);
@@ -475,8 +475,8 @@ constexpr LaneSwap make_lane_swap(int L, int E, int S) {
         int sg = r0;
         if (a >= 0) sg |= ((t0 >> 5) & 1) << a;
         if (c >= 0) sg |= ((t0 >> 4) & 1) << c;
-        if ((used >> sg) & 1) { s.ok = false; break; }
-        used |= 1 << sg;
+        if ((used >> sg) & 1ull) { s.ok = false; break; }
+        used |= 1ull << sg;
         s.sigma[e] = sg;
         for (int t = 0; t < T && s.ok; ++t) {
           int tp = 0, r = 0;
