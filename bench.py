@@ -27,7 +27,7 @@ METRIC = "z-slices/s at 1024² CF32, % of HBM roofline; p50/p99 per-hologram lat
 HBM_PEAK_GBS = 8000.0
 LAM, DX = 0.6328, 10.0
 N, SLICES, Z0, DZ, NDISKS, SEED = 1024, 1000, 1e4, 200.0, 1000, 102
-# Traffic models, bytes per pixel and slice (DESIGN.md, section 3):
+# Traffic models, bytes per pixel and slice:
 #   fused (this design, the roofline's algorithmic bytes): pass A writes the column-transformed
 #   slice (8) and reads the spectrum once per slice group of spg_gen slices (8 / spg_gen); pass B
 #   reads the slice (8) and read-modify-writes the projection once per group of spg_out slices
@@ -252,10 +252,10 @@ def main():
         cpu = cpu_baseline.measure(holo_np, -Z0, -DZ, LAM, DX, SLICES, budget_s=15.0)
         cpu["value"] = round(cpu["value"], 2)
         # the CPU configuration: 512^2, seeded 200-disk scene, 100 slices at 1e3 um from 1e4 um
-        c1_holo = synth.hologram(101, 200, 512, zmin=1e4, zmax=1e5, wavlen=LAM, dx=DX)
-        c1 = cpu_baseline.measure(c1_holo, -1e4, -1e3, LAM, DX, 100, budget_s=6.0)
-        cpu["c1_512x512x100"] = {"value": round(c1["value"], 2), "unit": c1["unit"], "sample": c1["sample"]}
-        # checker: the measured projection of the timed workload against the FP64 oracle
+        small_holo = synth.hologram(101, 200, 512, zmin=1e4, zmax=1e5, wavlen=LAM, dx=DX)
+        small = cpu_baseline.measure(small_holo, -1e4, -1e3, LAM, DX, 100, budget_s=6.0)
+        cpu["seeded_512x512x100"] = {"value": round(small["value"], 2), "unit": small["unit"], "sample": small["sample"]}
+        # the measured projection of the timed workload against the FP64 oracle
         log("bench: FP64 oracle of the 1024^2 x 1000 workload (parity of the measured projection)")
         want = asm.reconst_xyprojection_running(asm.gabor_wavefront(holo_np), SLICES, z_front=-Z0, z_dz=-DZ,
                                                 wavlen=LAM, dx=DX)

@@ -1,7 +1,7 @@
 """CPU oracle for the angular-spectrum / Gerchberg-Saxton path (test infrastructure only).
 
-This module is the checker. Only tests, the smoke check and the CPU-baseline leg of the
-benchmark may import it; the product path never calls it.
+Only tests, the smoke check and the CPU-baseline leg of the benchmark may import it; the
+product path never calls it.
 
 Array convention: numpy arrays are C-ordered with shape ``(ny, nx)``. Element ``[j, i]``
 is the reference's Julia element ``[i+1, j+1]`` (Julia dim 1 = ``x`` = the contiguous axis),

@@ -4,14 +4,14 @@ infrastructure, CPU only; prints a markdown table).
 The reference builds d in FP64 and stores it as Float32, takes the Float32 sqrt before its FP64
 exp (holofunc.jl:12, 42), rounds H to ComplexF32 and advances the spectrum by the ComplexF32
 running product F .*= Hdz (holofunc.jl:260). ``asm`` restates that as its 'reference' mode; this
-script compares it with the FP64 'exact' evaluation on the seeded configured inputs:
+script compares it with the FP64 'exact' evaluation on the seeded inputs:
 
-  C1  512^2, seed-101 200-disk hologram, 100 slices from z0 = 1e4 um at dz = 1e3 um
-  C2  1024^2, seed-102 1000-disk hologram, 1000 slices from 1e4 um at 200 um
-  C3  1024^2, seed-103 500-disk two-plane pair 2000 um apart, GS 100 iterations (transfer arrays
+  512^2, seed-101 200-disk hologram, 100 slices from z0 = 1e4 um at dz = 1e3 um
+  1024^2, seed-102 1000-disk hologram, 1000 slices from 1e4 um at 200 um
+  1024^2, seed-103 500-disk two-plane pair 2000 um apart, GS 100 iterations (transfer arrays
       in each arithmetic), then 512 slices from 1e4 um at 200 um
 
-Run: python3 -m oracle.emulation_deviation [--quick]   (--quick: C1 only)
+Run: python3 -m oracle.emulation_deviation [--quick]   (--quick: the 512^2 case only)
 """
 from __future__ import annotations
 
@@ -74,10 +74,10 @@ def main(argv=None):
     from particleholography import synth
 
     rows = []
-    configs = [("C1 512^2 x 100, dz 1e3", 101, 200, 512, 100, -1e4, -1e3, 1e5)]
+    cases = [("512^2 x 100, dz 1e3", 101, 200, 512, 100, -1e4, -1e3, 1e5)]
     if not args.quick:
-        configs.append(("C2 1024^2 x 1000, dz 200", 102, 1000, 1024, 1000, -1e4, -200.0, 2e5))
-    for name, seed, nd, n, slices, z0, dz, zmax in configs:
+        cases.append(("1024^2 x 1000, dz 200", 102, 1000, 1024, 1000, -1e4, -200.0, 2e5))
+    for name, seed, nd, n, slices, z0, dz, zmax in cases:
         holo = synth.hologram(seed, nd, n, zmin=1e4, zmax=zmax, wavlen=LAM, dx=DX)
         wf = asm.gabor_wavefront(holo)
         exact = asm.reconst_xyprojection_running(wf, slices, z_front=z0, z_dz=dz, wavlen=LAM, dx=DX)
@@ -95,7 +95,7 @@ def main(argv=None):
         exact = asm.reconst_xyprojection_running(psi_exact.astype(np.complex64), slices, z_front=-1e4, z_dz=-200.0,
                                                  wavlen=LAM, dx=DX)
         ref = reference_projection(psi_ref.astype(np.complex64), slices, -1e4, -200.0)
-        rows.append((f"C3 1024^2 GS {iters} it + {slices} slices (GS phase RMS {gs_rms:.2e} rad)",
+        rows.append((f"1024^2 GS {iters} it + {slices} slices (GS phase RMS {gs_rms:.2e} rad)",
                      f"1e4-{1e4 + (slices - 1) * 200:.0f}", asm.rel_l2(ref, exact), transfer_phase_error(n, 1e4 + (slices - 1) * 200)))
     print("| config | depths (um) | projection rel L2, reference arithmetic vs FP64 | H phase error at the deepest slice, max / RMS (rad) |")
     print("|---|---|---|---|")

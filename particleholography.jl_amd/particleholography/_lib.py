@@ -110,7 +110,7 @@ def load() -> ctypes.CDLL:
             return _lib
         if not LIB_PATH.exists():
             raise ImportError(f"libholo.so not found at {LIB_PATH}; build it with "
-                              f"`make -C {PKG_ROOT / 'csrc'}` (or __graft_entry__.build())")
+                              f"`make -C {PKG_ROOT / 'csrc'}`")
         lib = ctypes.CDLL(str(LIB_PATH))
         for name, args in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -213,7 +213,7 @@ def test_projection_non_square(gpu, nx, ny):
     assert asm.rel_l2(got, want) < 1e-4
 
 
-def test_projection_c1_config_vs_fp64_oracle(gpu):
+def test_projection_512_seeded_100_slices_vs_fp64_oracle(gpu):
     """512^2, 100 slices, z0 = 1e4, dz = 1e3 um on a seeded disk hologram (the CPU config)."""
     ph = _ph()
     from particleholography import synth
@@ -225,11 +225,11 @@ def test_projection_c1_config_vs_fp64_oracle(gpu):
     want = asm.reconst_xyprojection(asm.gabor_wavefront(holo), 100, "exact", z_front=-1e4, z_dz=-1e3,
                                     wavlen=LAM, dx=DX)
     err = asm.rel_l2(got, want)
-    print(f"C1 512^2x100 projection rel L2 vs FP64 oracle: {err:.3e}")
+    print(f"512^2 x 100 seeded projection rel L2 vs FP64 oracle: {err:.3e}")
     assert err < 1e-4
 
 
-def test_projection_c2_full_workload_vs_fp64_oracle(gpu):
+def test_projection_1024_seeded_1000_slices_vs_fp64_oracle(gpu):
     """The headline workload end to end: the seed-102 1000-disk 1024^2 hologram, 1000 slices at
     dz = 200 um from z0 = 1e4 um (to 2.098e5 um, holofunc.jl:259-263), through the fused path
     (sqrt fused into the forward FFT, the shipped tiles), against the FP64 oracle."""
@@ -244,11 +244,11 @@ def test_projection_c2_full_workload_vs_fp64_oracle(gpu):
     want = asm.reconst_xyprojection_running(asm.gabor_wavefront(holo), slices, z_front=-1e4, z_dz=-200.0,
                                             wavlen=LAM, dx=DX)
     err = asm.rel_l2(got, want)
-    print(f"C2 1024^2x1000 projection rel L2 vs FP64 oracle: {err:.3e} (max abs {np.abs(got - want).max():.3e})")
+    print(f"1024^2 x 1000 seeded projection rel L2 vs FP64 oracle: {err:.3e} (max abs {np.abs(got - want).max():.3e})")
     assert err < 1e-4
 
 
-def test_gs_c3_config_vs_fp64_oracle(gpu):
+def test_gs_1024_seeded_pair_100_iterations_vs_fp64_oracle(gpu):
     """The GS configuration: the seeded 500-disk scene recorded at planes 2000 um apart
     (1024^2), 100 iterations on the production path (graphs, line shapes), against the FP64
     oracle's 100 iterations: amplitude-weighted phase RMS."""
@@ -263,7 +263,7 @@ def test_gs_c3_config_vs_fp64_oracle(gpu):
     want = asm.phase_retrieval(i1, i2, iters, z=z, wavlen=LAM, dx=DX)
     np.testing.assert_allclose(np.abs(out), np.sqrt(i1), rtol=2e-6)
     err = asm.weighted_phase_rms(out, want)
-    print(f"C3 GS 1024^2 x 100 iterations: weighted phase RMS vs FP64 oracle {err:.2e} rad")
+    print(f"GS 1024^2 seeded pair x 100 iterations: weighted phase RMS vs FP64 oracle {err:.2e} rad")
     assert err < 1e-3
 
 
