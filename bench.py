@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--profile-detect", action="store_true",
                     help="for rocprofv3: the particle-detection extra (1024^2 x 1000 N0f8 volume, "
                          "labels and rectangles of every slice, --steps calls); no other work")
+    ap.add_argument("--profile-ingest", action="store_true",
+                    help="for rocprofv3: 8-bit ingest of 64 1024^2 holograms and the background mode / mean of a "
+                         "256-image stack (--steps calls of each after warm-up); no other work")
     ap.add_argument("--profile-batch", action="store_true",
                     help="for rocprofv3: the 64-hologram batched reconstruction (shipped tiles, "
                          "--steps calls after one warm-up); no other work")
@@ -121,6 +124,11 @@ def main():
         r = bench_detect(ph, torch, dev, reps=args.steps)
         log("bench: detection profile run done")
         print(json.dumps({"profile_detect": True, "steps": args.steps, **r}), flush=True)
+        return
+    if args.profile_ingest:
+        r = bench_ingest(ph, torch, dev, reps=args.steps)
+        log("bench: ingest profile run done")
+        print(json.dumps({"profile_ingest": True, "steps": args.steps, **r}), flush=True)
         return
     if args.profile_batch:
         r = bench_batched(torch, dev, args.batch_holograms, autotune=False, reps=args.steps)
@@ -352,7 +360,7 @@ def bench_lpf(ph, torch, holo):
             "config": "1024^2 ComplexF32 wavefront, order-3 super-Gaussian cutoff for 2.1e5 um"}
 
 
-def bench_ingest(ph, torch, dev):
+def bench_ingest(ph, torch, dev, reps=20):
     """8-bit ingest at 1024^2: 64 device-resident uint8 holograms converted to Float32 with the
     background removal fused (1 B in + 8 B background + 4 B out per pixel), and the per-pixel
     background mode / mean of a 256-image stack (1 B per pixel per image, read once)."""
@@ -360,7 +368,6 @@ def bench_ingest(ph, torch, dev):
     stack = torch.randint(0, 256, (256, N, N), dtype=torch.uint8, generator=g).to(dev)
     bkg = torch.rand((N, N), dtype=torch.float64, generator=g).to(dev)
     batch = stack[:64]
-    reps = 20
     _, t_ing = timed_steps(lambda: ph.cu_ingest_u8(batch, bkg=bkg), reps, 2, torch)
     _, t_mode = timed_steps(lambda: ph.cu_make_background(stack, "mode"), reps, 2, torch)
     _, t_mean = timed_steps(lambda: ph.cu_make_background(stack, "mean"), reps, 2, torch)
