@@ -122,6 +122,8 @@ struct holo_context {
   int gs_xs = 1;                 // single-plane column passes over 1024-tall columns: HOLO_GS_XS 0 the
                                  // plane tiles' 8 elements per thread, 1 / 2 16 elements with lane-swap
                                  // exchanges and one / two LDS buffers
+  int gblk_w = 4;                // widest pass-A tile whose slices are stored in 4 x 4 blocks
+                                 // (HOLO_GBLK_W; 0 keeps every slice row-major)
   std::map<std::tuple<const void*, int, size_t>, int> occ;  // resident workgroups per CU, by kernel
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -421,6 +423,9 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
   if (rows * px->lp.T > 512 || rows * rowbytes + rowtw > maxl)
     return fail(HOLO_ERR_ARG, "row tile too large for a workgroup");
   g->rows = rows;
+  // the reconstruction's slices in 4 x 4 blocks where pass A's tiles are at most 4 columns wide
+  // (their row segments would be 32-byte pieces of 128-byte lines)
+  g->gblk = (W <= h->gblk_w && nx % 4 == 0 && ny % 4 == 0 && px->lp.T % 4 == 0 && py->lp.T % 4 == 0) ? 1 : 0;
   return HOLO_OK;
 }
 
@@ -1085,6 +1090,7 @@ int holo_create(holo_handle* out, int device) {
   h->col_db = static_cast<int>(env_size("HOLO_COL_DB", 1));
   h->row_pf = static_cast<int>(env_size("HOLO_ROW_PF", 1));
   h->gs_xs = static_cast<int>(env_size("HOLO_GS_XS", 1));
+  h->gblk_w = static_cast<int>(env_size("HOLO_GBLK_W", 4));
   // Per-size tiles chosen by holo_autotune on an MI355X and confirmed on single-hologram
   // reconstructions (768^2 and 1280x1024 +37%, 512^2 +5% over the generic rule). 2048^2 comes
   // from a direct sweep: no useful round fits the 256 MB Infinity Cache at that size, so long

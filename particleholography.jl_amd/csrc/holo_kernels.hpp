@@ -26,7 +26,19 @@ struct Geom {
   int W;        // columns per workgroup in column passes (power of two)
   int wshift;   // log2(W)
   int rows;     // rows per workgroup in row passes
+  int gblk;     // the reconstruction's column-transformed slices G are stored in 4 x 4 blocks
 };
+
+// Element (row j, column i) of a reconstruction slice G: natural order j * nx + i, or, with
+// gblk, in blocks of 4 rows x 4 columns (16 consecutive elements, one 128-byte line), so that a
+// column pass over 4-column tiles writes whole lines instead of 32-byte pieces of four
+// (2048^2 pass A: 2.1 -> 1.6 ms per 192 slices). Needs nx and ny multiples of 4, and threads
+// per line (both passes) multiples of 4: element t + T*e of a line then sits at the thread's
+// base plus T*nx*e (columns) or 4*T*e (rows).
+__device__ __forceinline__ unsigned g_index(const Geom& g, unsigned j, unsigned i) {
+  return g.gblk ? ((j >> 2) * (static_cast<unsigned>(g.nx) >> 2) + (i >> 2)) * 16u + (j & 3u) * 4u + (i & 3u)
+                : j * static_cast<unsigned>(g.nx) + i;
+}
 
 // Parameters of the FP64 transfer-function generator (natural FFT order).
 //   a(n, N, c) = (((ic - N/2) * wavlen) / N / dx)^2 with ic = (n + N/2) mod N  (holofunc.jl:12)
@@ -190,6 +202,17 @@ __device__ __forceinline__ void twiddles_visible() {
   if constexpr (kWaveRow<E, L>) __syncthreads();
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2) by linear
+// workgroup id. Column tiles are renumbered so that each XCD walks a contiguous range of
+// tiles: neighbouring tiles, which share cache lines when W*8 bytes is below a line, meet in
+// the same L2. Row tiles of a blocked G (which share its 4-row blocks when a tile holds
+// fewer than 4 rows) are renumbered the same way.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n / kXcds;
+  return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
+}
+
 struct RowCtx {
   int t, w, j;
   int jl;  // j clamped into the plane: loads read row jl unconditionally (see below)
@@ -203,7 +226,7 @@ __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
   const int T = line_threads<E, L>(g.px);
   c.t = threadIdx.x % T;
   c.w = threadIdx.x / T;
-  c.j = blockIdx.x * g.rows + c.w;
+  c.j = (g.gblk ? xcd_tile(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * g.rows + c.w;
   c.ok = c.j < g.ny;
   c.jl = c.ok ? c.j : g.ny - 1;
   c.line = lds + c.w * lds_padded_len(g.nx, kRowPad);
@@ -222,15 +245,6 @@ struct ColCtx {
   float2* col;
   float2* tw;
 };
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2) by linear
-// workgroup id. Column tiles are renumbered so that each XCD walks a contiguous range of
-// tiles: neighbouring tiles, which share cache lines when W*8 bytes is below a line, meet in
-// the same L2.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-  const int q = n / kXcds;
-  return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
-}
 
 // Column-tile accesses: element (row t + T*e, column ci) of a plane at a uniform base. The row
 // term of each element folds into a scalar base pointer and the thread keeps one 32-bit offset
@@ -431,6 +445,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
   const bool ok = c.i < g.nx;
   const int ci = ok ? c.i : 0;
   const unsigned off = static_cast<unsigned>(c.t * g.nx + ci);
+  const unsigned offg = g_index(g, static_cast<unsigned>(c.t), static_cast<unsigned>(ci));  // G's element t, ci
   const unsigned pbytes = static_cast<unsigned>(static_cast<size_t>(g.nx) * g.ny * sizeof(float2));
   const unsigned rowb = static_cast<unsigned>(T * g.nx * sizeof(float2));
   float2 f[E];
@@ -487,7 +502,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
           for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
         }
         fft_col_xs<E, +1, L, TWR>(v, c.col + ((k & 1) ? bo : 0), W, c.t, c.tw, twr);
-        col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? off * 8u : pbytes, rowb);
+        col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? offg * 8u : pbytes, rowb);
       }
     } else {
     for (int k = k0; k < k1; ++k) {
@@ -504,7 +519,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
         fft_col<E, +1, L>(v, c.col, g.W, c.t, g.py, c.tw);
       }
       // columns past the plane (the last tile) get an offset past the buffer: dropped stores
-      col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? off * 8u : pbytes, rowb);
+      col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? offg * 8u : pbytes, rowb);
     }
     }
   } else {
@@ -536,7 +551,7 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
       if (ok) {
         float2* __restrict__ dst = a.G + static_cast<size_t>(k) * a.gstride;
 #pragma unroll
-        for (int e = 0; e < E; ++e) dst[static_cast<size_t>(c.t + T * e) * g.nx + c.i] = v[e];
+        for (int e = 0; e < E; ++e) dst[offg + static_cast<size_t>(T) * g.nx * e] = v[e];
       }
     }
     if (ok) {
@@ -588,6 +603,9 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
   if constexpr (TWR) load_tw_regs<E, L>(twr, c.t, g.twx);
   const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.jl) * g.nx;
+  // this thread's elements t + T*e of row jl in G: base + estride*e
+  const unsigned gbase = g_index(g, static_cast<unsigned>(c.jl), static_cast<unsigned>(c.t));
+  const unsigned estride = g.gblk ? 4u * T : static_cast<unsigned>(T);
   const int k0 = blockIdx.y * spg;
   const int k1 = min(k0 + spg, nbatch);
   unsigned mn[E];
@@ -605,12 +623,12 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
   if constexpr (PF) {
     // rows through a buffer resource: the slice's base in SGPRs, one 32-bit offset per thread
     const unsigned pbytes = static_cast<unsigned>(gstride * sizeof(float2));
-    const unsigned voff = static_cast<unsigned>((rowoff + c.t) * sizeof(float2));
+    const unsigned voff = gbase * static_cast<unsigned>(sizeof(float2));
     auto load = [&](float2 (&v)[E], int k) {
       const __amdgpu_buffer_rsrc_t r = plane_rsrc(G + static_cast<size_t>(k) * gstride, pbytes);
 #pragma unroll
       for (int e = 0; e < E; ++e)
-        v[e] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, e * T * 8, 0));
+        v[e] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, e * estride * 8, 0));
     };
     float2 a[E], b[E];
     load(a, k0);
@@ -625,9 +643,9 @@ __global__ __launch_bounds__(512) void k_recon_rows_proj(JobTable jt, size_t gst
   } else {
     for (int k = k0; k < k1; ++k) {
       float2 v[E];
-      const float2* __restrict__ src = G + static_cast<size_t>(k) * gstride + rowoff;
+      const float2* __restrict__ src = G + static_cast<size_t>(k) * gstride + gbase;
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = src[c.t + T * e];
+      for (int e = 0; e < E; ++e) v[e] = src[estride * e];
       fold(v);
     }
   }
@@ -710,7 +728,6 @@ __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
   extern __shared__ float2 lds[];
   const RowCtx c = row_ctx<E, L>(g, lds);
   const int T = line_threads<E, L>(g.px);
-  const size_t rowoff = static_cast<size_t>(c.j) * g.nx;
   const int oj = CROP ? c.j - a.oy : c.j;
   const bool rowin = CROP ? (c.ok && oj >= 0 && oj < a.ony) : c.ok;
   const int k0 = blockIdx.y * a.spg;
@@ -718,11 +735,14 @@ __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
   float mn[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) mn[e] = __uint_as_float(0x7fffffffu);
+  // this thread's elements t + T*e of row jl in G: base + estride*e
+  const unsigned gbase = g_index(g, static_cast<unsigned>(c.jl), static_cast<unsigned>(c.t));
+  const unsigned estride = g.gblk ? 4u * T : static_cast<unsigned>(T);
   for (int k = k0; k < k1; ++k) {
     float2 v[E];
-    const float2* __restrict__ src = a.G + static_cast<size_t>(k) * a.gstride + rowoff;
+    const float2* __restrict__ src = a.G + static_cast<size_t>(k) * a.gstride + gbase;
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = c.ok ? src[c.t + T * e] : make_float2(0.f, 0.f);
+    for (int e = 0; e < E; ++e) v[e] = c.ok ? src[estride * e] : make_float2(0.f, 0.f);
     fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
     if (rowin) {
       const size_t sbase = static_cast<size_t>(a.slice0 + k) * a.vstride + static_cast<size_t>(oj) * a.onx;

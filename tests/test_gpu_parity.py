@@ -156,7 +156,9 @@ def test_pass_a_exchange_variants_bitwise_equal(gpu, nx, ny):
     sides run their two-buffer form for those settings). The projection is bitwise the same for
     each, at every tile width up to 512 threads, over a slice count that leaves a short last
     group (odd slice counts per group use both buffers first), and within the bar of the FP64
-    oracle."""
+    oracle. Tiles of 2 and 4 columns store the intermediate slices in 4 x 4 blocks and 8-column
+    tiles row-major, so the comparison across widths also covers the two layouts of pass A's
+    stores and pass B's loads."""
     import ctypes
 
     from particleholography import _lib, batch
