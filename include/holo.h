@@ -151,7 +151,8 @@ int holo_reconst_vol_and_xyprojection_padded(holo_handle h, void* stream, int nx
  * psi1 = P_bwd psi2, psi1 <- sqrt(I1) psi1/|psi1|. out (ComplexF32 nx*ny) = psi1. With
  * use_graph != 0 the identical iterations run as hipGraphs of up to 16 iterations, captured
  * once per size and tile choice and cached on the handle (they reference only its workspace,
- * into which I1 and I2 are copied). */
+ * into which the amplitudes sqrt(I1) and sqrt(I2) are written, as holofunc.jl:107-108 takes them
+ * once per call). */
 int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const float* holo1,
                          const float* holo2, const holo_tf* fwd, const holo_tf* bwd,
                          int iters, void* out, int use_graph);
@@ -278,8 +279,10 @@ typedef struct holo_gs_job {
 int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* jobs, int njobs);
 /* Batched GS runs the iterations that are identical for a chunk of pairs (all but each pair's
  * last) as hipGraphs of up to 16 iterations, captured once per size, chunk layout and workspace
- * and cached on the handle (I1 and I2 are copied into the workspace, so the graphs reference no
- * caller memory); enable = 0 launches every pass directly. No counterpart in the reference. */
+ * and cached on the handle (the amplitudes sqrt(I1), sqrt(I2) are written into the workspace, so
+ * the graphs reference no caller memory); enable = 0 launches every pass directly. A chunk holds
+ * as many pairs as keep its fields and amplitudes (16 B/px) within 256 MiB, at most 32
+ * (HOLO_GS_CHUNK sets the count). No counterpart in the reference. */
 int holo_set_gs_graphs(holo_handle h, int enable);
 /* Caps the workspace the batched calls plan for (holograms are processed in chunks that fit;
  * a single hologram larger than the cap still runs). 0 = HOLO_BATCH_WS_BYTES or 8 GiB. */

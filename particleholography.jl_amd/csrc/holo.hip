@@ -146,9 +146,11 @@ struct holo_context {
     int n;       // iterations captured
     int npairs;  // batched calls: pairs of the chunk (0 for the single-pair call)
     unsigned long long conjmask;  // batched calls: pairs whose backward table is the conjugate
+    unsigned long long hqmask;    // bits 2q, 2q + 1: pair q's forward / backward table is a quadrant
     bool operator<(const GsKey& o) const {
-      return std::tie(nx, ny, buf, hb, conj, W, rows, ex, ey, n, npairs, conjmask) <
-             std::tie(o.nx, o.ny, o.buf, o.hb, o.conj, o.W, o.rows, o.ex, o.ey, o.n, o.npairs, o.conjmask);
+      return std::tie(nx, ny, buf, hb, conj, W, rows, ex, ey, n, npairs, conjmask, hqmask) <
+             std::tie(o.nx, o.ny, o.buf, o.hb, o.conj, o.W, o.rows, o.ex, o.ey, o.n, o.npairs, o.conjmask,
+                      o.hqmask);
     }
   };
   std::map<GsKey, hipGraphExec_t> gs_graphs;
@@ -969,8 +971,9 @@ struct GsPair {
   const float2* hf;
   const float2* hb;
   int conj_b;
-  const float* I1;
-  const float* I2;
+  int hq_f, hq_b;  // hf / hb hold the quadrant of an even-sided generated transfer
+  const float* A1;  // amplitudes sqrt(I1), sqrt(I2) in the workspace
+  const float* A2;
   float2* out;
   int iters;
 };
@@ -984,11 +987,11 @@ int gs_round(holo_context* h, const Geom& g, int Ex, int Ey, const GsPair* pairs
     const GsPair& p = pairs[q];
     if (p.iters <= it) continue;
     const int k = f.n;
-    f.d[k] = GsDesc{p.B, p.hf, nullptr, nullptr, 0, 0};
-    r2.d[k] = GsDesc{p.B, nullptr, p.I2, nullptr, 0, 0};
-    b.d[k] = GsDesc{p.B, p.hb, nullptr, nullptr, p.conj_b, 0};
+    f.d[k] = GsDesc{p.B, p.hf, nullptr, nullptr, 0, 0, p.hq_f};
+    r2.d[k] = GsDesc{p.B, nullptr, p.A2, nullptr, 0, 0, 0};
+    b.d[k] = GsDesc{p.B, p.hb, nullptr, nullptr, p.conj_b, 0, p.hq_b};
     const bool last = it + 1 == p.iters;
-    r1.d[k] = GsDesc{p.B, nullptr, p.I1, last ? p.out : nullptr, 0, last ? 1 : 0};
+    r1.d[k] = GsDesc{p.B, nullptr, p.A1, last ? p.out : nullptr, 0, last ? 1 : 0, 0};
     f.n = r2.n = b.n = r1.n = k + 1;
   }
   if (f.n == 0) return HOLO_OK;
@@ -1005,29 +1008,52 @@ int gs_round(holo_context* h, const Geom& g, int Ex, int Ey, const GsPair* pairs
   return HOLO_OK;
 }
 
-// Natural-order transfer arrays of a pair into hf (and hb unless mirrored): generated
+// The amplitudes sqrt(I) a GS call imposes, into its workspace: the graphs then reference
+// workspace addresses only, whatever the caller passes.
+int launch_sqrt_plane(const void* in, float* out, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_sqrt_plane, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, n,
+                     static_cast<const float*>(in), out);
+  return check_launch();
+}
+
+// Natural-order transfer arrays of a pair into hf (and hbuf unless mirrored): generated
 // transfers are evaluated once (H(-z) = conj H(z) for a mirrored pair), arrays are ifftshifted.
-int gs_transfers(const Geom& g, const holo_tf* fwd, const holo_tf* bwd, float2* hf, float2* hbuf,
-                 const float2** hb, int* conj_b, hipStream_t st) {
-  const dim3 grid((g.nx + 255) / 256, g.ny), block(256);
+// A generated transfer of even sides is stored as its quadrant (nx/2 + 1) x (ny/2 + 1), which
+// determines the rest (k_transfer_natural): the column pass reads 2 MB instead of 8 at 1024^2,
+// and a batch's tables stay in the caches.
+int gs_transfers(const Geom& g, const holo_tf* fwd, const holo_tf* bwd, GsPair* p, float2* hf, float2* hbuf,
+                 hipStream_t st) {
+  const dim3 block(256);
   const bool mirrored = fwd->generated && bwd->generated && bwd->z == -fwd->z &&
                         bwd->wavlen == fwd->wavlen && bwd->dx == fwd->dx;
+  const bool even = g.nx % 2 == 0 && g.ny % 2 == 0;
   auto make = [&](const holo_tf* t, float2* dst) {
+    if (t->generated && even) {
+      const int w = g.nx / 2 + 1;
+      hipLaunchKernelGGL(k_transfer_natural, dim3((w + 255) / 256, g.ny / 2 + 1), block, 0, st, g.nx, g.ny, w,
+                         t->z / t->wavlen, t->wavlen, t->dx, dst);
+      return 1;
+    }
+    const dim3 grid((g.nx + 255) / 256, g.ny);
     if (t->generated)
-      hipLaunchKernelGGL(k_transfer_natural, grid, block, 0, st, g.nx, g.ny, t->z / t->wavlen, t->wavlen, t->dx, dst);
+      hipLaunchKernelGGL(k_transfer_natural, grid, block, 0, st, g.nx, g.ny, g.nx, t->z / t->wavlen, t->wavlen,
+                         t->dx, dst);
     else
       hipLaunchKernelGGL(k_to_natural, grid, block, 0, st, g.nx, g.ny, static_cast<const float2*>(t->data), dst);
+    return 0;
   };
-  make(fwd, hf);
+  p->hf = hf;
+  p->hq_f = make(fwd, hf);
   HOLO_TRY(check_launch());
   if (mirrored) {
-    *hb = hf;
-    *conj_b = 1;
+    p->hb = p->hf;
+    p->hq_b = p->hq_f;
+    p->conj_b = 1;
   } else {
-    make(bwd, hbuf);
+    p->hq_b = make(bwd, hbuf);
     HOLO_TRY(check_launch());
-    *hb = hbuf;
-    *conj_b = 0;
+    p->hb = hbuf;
+    p->conj_b = 0;
   }
   return HOLO_OK;
 }
@@ -1740,23 +1766,23 @@ int holo_phase_retrieval(holo_handle h, void* stream, int nx, int ny, const floa
   GsPair p{};
   p.B = static_cast<float2*>(h->ws);
   float2* hf = p.B + plane;
-  float* i1c = reinterpret_cast<float*>(hf + 2 * plane);
-  float* i2c = i1c + plane;
-  HOLO_HIP(hipMemcpyAsync(i1c, holo1, plane * sizeof(float), hipMemcpyDeviceToDevice, st));
-  HOLO_HIP(hipMemcpyAsync(i2c, holo2, plane * sizeof(float), hipMemcpyDeviceToDevice, st));
-  p.I1 = i1c;
-  p.I2 = i2c;
+  float* a1 = reinterpret_cast<float*>(hf + 2 * plane);
+  float* a2 = a1 + plane;
+  HOLO_TRY(launch_sqrt_plane(holo1, a1, plane, st));
+  HOLO_TRY(launch_sqrt_plane(holo2, a2, plane, st));
+  p.A1 = a1;
+  p.A2 = a2;
   p.out = static_cast<float2*>(out);
   p.iters = iters;
-  p.hf = hf;
-  HOLO_TRY(gs_transfers(g, fwd, bwd, hf, hf + plane, &p.hb, &p.conj_b, st));
+  HOLO_TRY(gs_transfers(g, fwd, bwd, &p, hf, hf + plane, st));
   const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
   HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, holo1, 0, p.B, 0, 1.0f, 1, st));  // psi1 = sqrt(I1), row FFT
   const int body = iters - 1;  // iterations 0 .. iters-2 are identical; the last one stores psi
   if (use_graph && body > 0) {
     // A graph holds up to kGsGraphIters iterations (4 launches each): graphs of one iteration
     // leave the device idle between graph launches.
-    holo_context::GsKey key{nx, ny, p.B, p.hb, p.conj_b, g.W, g.rows, Ex, Ey, 0, 0, 0};
+    holo_context::GsKey key{nx, ny, p.B, p.hb, p.conj_b, g.W, g.rows, Ex, Ey, 0, 0, 0,
+                            static_cast<unsigned long long>(p.hq_f | p.hq_b << 1)};
     const int chunk = std::min(body, kGsGraphIters);
     hipGraphExec_t full = nullptr, rest = nullptr;
     HOLO_TRY(gs_graph(h, key, chunk, g, Ex, Ey, &p, sc, &full));
@@ -1804,36 +1830,42 @@ int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* j
     gt.rows = h->tune.rows;
     HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &gt));
     const size_t plane = static_cast<size_t>(nx) * ny;
-    // per pair: field, forward and backward tables, copies of I1 and I2 (so that captured
-    // iterations reference workspace addresses only)
+    // per pair: field, forward and backward tables, the amplitudes sqrt(I1), sqrt(I2) (so that
+    // captured iterations reference workspace addresses only)
     const size_t perpair = plane * (3 * sizeof(float2) + 2 * sizeof(float));
-    const size_t chunk = std::max<size_t>(1, std::min<size_t>(kMaxJobs, budget / perpair));
+    size_t chunk = std::max<size_t>(1, std::min<size_t>(kMaxJobs, budget / perpair));
+    // Pairs per launch: as many as keep the fields and amplitudes (16 B/px) of the chunk within
+    // 256 MiB, so that each pass finds most of what the previous one wrote in the Infinity
+    // Cache (1024^2, 32 pairs: 54.8k pair-iterations/s in one chunk of 32, 63.0k in chunks of
+    // 16, 58.6k in chunks of 8); HOLO_GS_CHUNK sets the count instead.
+    const size_t cap = env_size("HOLO_GS_CHUNK", 0);
+    chunk = std::min(chunk, cap > 0 ? cap : std::max<size_t>(1, (256ull << 20) / (plane * 16)));
     const float sc = static_cast<float>(1.0 / (static_cast<double>(nx) * ny));
     for (size_t c0 = g0; c0 < g1; c0 += chunk) {
       const size_t c1 = std::min(g1, c0 + chunk);
       HOLO_TRY(ensure_ws(h, perpair * (c1 - c0)));
       std::vector<GsPair> pairs(c1 - c0);
       int maxit = 0, minit = std::numeric_limits<int>::max();
-      unsigned long long conjmask = 0;
+      unsigned long long conjmask = 0, hqmask = 0;
       for (size_t q = c0; q < c1; ++q) {
         const holo_gs_job& j = jobs[order[q]];
         GsPair& p = pairs[q - c0];
         p.B = reinterpret_cast<float2*>(static_cast<char*>(h->ws) + perpair * (q - c0));
         float2* hf = p.B + plane;
-        p.hf = hf;
-        HOLO_TRY(gs_transfers(g, &j.fwd, &j.bwd, hf, hf + plane, &p.hb, &p.conj_b, st));
-        float* i1c = reinterpret_cast<float*>(hf + 2 * plane);
-        float* i2c = i1c + plane;
-        HOLO_HIP(hipMemcpyAsync(i1c, j.holo1, plane * sizeof(float), hipMemcpyDeviceToDevice, st));
-        HOLO_HIP(hipMemcpyAsync(i2c, j.holo2, plane * sizeof(float), hipMemcpyDeviceToDevice, st));
-        p.I1 = i1c;
-        p.I2 = i2c;
+        HOLO_TRY(gs_transfers(g, &j.fwd, &j.bwd, &p, hf, hf + plane, st));
+        float* a1 = reinterpret_cast<float*>(hf + 2 * plane);
+        float* a2 = a1 + plane;
+        HOLO_TRY(launch_sqrt_plane(static_cast<const float*>(j.holo1), a1, plane, st));
+        HOLO_TRY(launch_sqrt_plane(static_cast<const float*>(j.holo2), a2, plane, st));
+        p.A1 = a1;
+        p.A2 = a2;
         p.out = static_cast<float2*>(j.out);
         p.iters = j.iters;
         HOLO_TRY(launch_rows_fft(g, Ex, -1, 1, j.holo1, 0, p.B, 0, 1.0f, 1, st));
         maxit = std::max(maxit, j.iters);
         minit = std::min(minit, j.iters);
         if (p.conj_b) conjmask |= 1ull << (q - c0);
+        hqmask |= static_cast<unsigned long long>(p.hq_f | p.hq_b << 1) << (2 * (q - c0));
       }
       // iterations 0 .. minit-2 are identical for the chunk (every pair active, none storing its
       // result): graphs of up to kGsGraphIters of them, captured once per chunk layout; the rest
@@ -1841,7 +1873,7 @@ int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* j
       const int body = h->gs_batch_graphs ? minit - 1 : 0;
       if (body > 0) {
         holo_context::GsKey key{nx, ny, h->ws, nullptr, 0, g.W, g.rows, Ex, Ey, 0,
-                                static_cast<int>(pairs.size()), conjmask};
+                                static_cast<int>(pairs.size()), conjmask, hqmask};
         const int gchunk = std::min(body, kGsGraphIters);
         hipGraphExec_t full = nullptr, rest = nullptr;
         const int np = static_cast<int>(pairs.size());
@@ -2063,7 +2095,7 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
   };
   // a synthetic spectrum: any field of the right size (the kernels' cost does not depend on it)
   HOLO_TRY(ensure_ws(h, pbytes * 2));
-  hipLaunchKernelGGL(k_transfer_natural, dim3((nx + 255) / 256, ny), dim3(256), 0, st, nx, ny, 1.0e3 / wavlen,
+  hipLaunchKernelGGL(k_transfer_natural, dim3((nx + 255) / 256, ny), dim3(256), 0, st, nx, ny, nx, 1.0e3 / wavlen,
                      wavlen, dx, static_cast<float2*>(h->ws));
   HOLO_TRY(check_launch());
   Tuning best;

@@ -130,15 +130,25 @@ __global__ void k_transfer(int nx, int ny, int mode, double two_pi_z_over_l, dou
   out[idx] = h;
 }
 
-// Natural-order (ifftshifted) H(z) from FP64-wrapped phases; evanescent components 0.
-__global__ void k_transfer_natural(int nx, int ny, double zl, double wavlen, double dx,
+// Natural-order (ifftshifted) H(z) from FP64-wrapped phases; evanescent components 0. Writes
+// columns i < w of rows j < gridDim.y with row stride w: w = nx over ny rows for the whole
+// array, or the quadrant i <= nx/2, j <= ny/2 of an even-sided one. H depends on each axis
+// through the square of (ic - N/2), and for even N the natural indices n and N - n carry
+// opposite values of it (exactly, in FP64), so H(i, j) = H(min(i, nx - i), min(j, ny - j)).
+__global__ void k_transfer_natural(int nx, int ny, int w, double zl, double wavlen, double dx,
                                    float2* __restrict__ out) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
-  if (i >= nx) return;
+  if (i >= w) return;
   const double d = (1.0 - axis_a(i, nx, wavlen, dx)) - axis_a(j, ny, wavlen, dx);
-  out[static_cast<size_t>(j) * nx + i] = d >= 0.0 ? cis_cycles(zl * sqrt(d)) : make_float2(0.f, 0.f);
+  out[static_cast<size_t>(j) * w + i] = d >= 0.0 ? cis_cycles(zl * sqrt(d)) : make_float2(0.f, 0.f);
+}
+
+// sqrt.(holo): a GS plane's amplitude, taken once per call (holofunc.jl:107-108)
+__global__ void k_sqrt_plane(size_t n, const float* __restrict__ in, float* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = sqrtf(in[i]);
 }
 
 // psi = ComplexF32(sqrt(holo) + 0im)  (holofunc.jl:80-82)
@@ -777,10 +787,12 @@ __global__ __launch_bounds__(512) void k_recon_rows_store(StoreArgs a, Geom g) {
 struct GsDesc {
   float2* B;          // the pair's field (transformed in place)
   const float2* H;    // natural-order transfer array of this propagation
-  const float* I;     // intensity whose amplitude is imposed by the row pass
+  const float* A;     // amplitude sqrt(I) imposed by the row pass (holofunc.jl:107-108)
   float2* out;        // the retrieved wavefront (row pass with final = 1)
   int conj;           // multiply by conj(H)
   int final;          // row pass of the last iteration: store psi instead of its row FFT
+  int hq;             // H holds only the quadrant i <= nx/2, j <= ny/2 (row stride nx/2 + 1) of
+                      // an even-sided generated transfer (see k_transfer_natural)
 };
 struct GsTable {
   int n;
@@ -824,7 +836,11 @@ __global__ __launch_bounds__(512) void k_prop_cols(PropArgs a, GsTable gt, Geom 
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     if constexpr (HMODE == 1) hv[e] = a.H[centred_index(ci, c.t + T * e, g.nx, g.ny)];
-    if constexpr (HMODE == 2) hv[e] = pd.H[static_cast<size_t>(c.t + T * e) * g.nx + ci];
+    if constexpr (HMODE == 2) {
+      const int r = c.t + T * e;
+      hv[e] = pd.hq ? pd.H[min(r, g.ny - r) * ((g.nx >> 1) + 1) + min(ci, g.nx - ci)]
+                    : pd.H[static_cast<size_t>(r) * g.nx + ci];
+    }
     if constexpr (HMODE == 3) lv[e] = a.lpf[centred_index(ci, c.t + T * e, g.nx, g.ny)];
   }
   stage_twiddles(c.tw, g.twy, g.py.ntw);
@@ -875,25 +891,25 @@ __global__ __launch_bounds__(512) void k_gs_rows(GsTable gt, float sc, Geom g) {
   extern __shared__ float2 lds[];
   const GsDesc& rd = gt.d[blockIdx.z];
   const float2* __restrict__ U = rd.B;
-  const float* __restrict__ I = rd.I;
+  const float* __restrict__ Am = rd.A;
   const bool final = rd.final != 0;
   float2* __restrict__ out = final ? rd.out : rd.B;
   const RowCtx c = row_ctx<E, L, false>(g, lds);
   const int T = line_threads<E, L>(g.px);
   const size_t rowoff = static_cast<size_t>(c.jl) * g.nx;
   float2 v[E];
-  float iv[E];  // the intensity is loaded with the row, in the same memory latency
+  float av[E];  // the amplitude is loaded with the row, in the same memory latency
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     v[e] = U[rowoff + c.t + T * e];
-    iv[e] = I[rowoff + c.t + T * e];
+    av[e] = Am[rowoff + c.t + T * e];
   }
   stage_twiddles(c.tw, g.twx, g.px.ntw);
   twiddles_visible<E, L>();
   fft_row<E, +1, L>(v, c.line, c.t, g.px, c.tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const float A = sqrtf(iv[e]);
+    const float A = av[e];
     const float a = v[e].x * sc, b = v[e].y * sc;
     const float r2 = a * a + b * b;
     if (r2 >= 1e-30f) {  // v_rsq_f32 (1 ulp) away from the denormal range

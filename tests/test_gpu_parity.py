@@ -476,6 +476,25 @@ def test_gs_graphs_reused_across_inputs_and_iteration_counts(gpu):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("n", [128, 1024])
+def test_gs_quadrant_tables_equal_full_tables(gpu, n):
+    """Generated transfers of even sides go to the column pass as the quadrant
+    (n/2 + 1) x (n/2 + 1) of their natural-order table, read at min(i, n - i), min(j, n - j);
+    array-backed copies of the same transfers go as the whole table. Both hold the same values
+    (one FP64 generator), so the retrieved fields are equal bit for bit (a wrong mirror index
+    would shift H by one frequency step, about 0.04 rad per pass at 2000 um)."""
+    ph = _ph()
+    i1, i2 = _gs_case(n, 11, 2000.0)
+    dsq = ph.cu_transfer_sqrt_arr(n, LAM, DX)
+    gen = (ph.cu_transfer(2000.0, n, LAM, dsq), ph.cu_transfer(-1500.0, n, LAM, dsq))
+    arr = tuple(ph.CuTransfer(t.data.clone()) for t in gen)
+    assert gen[0].generated and not arr[0].generated
+    a = ph.cu_phase_retrieval_holo(_t(i1, gpu), _t(i2, gpu), *gen, 5, n).data.cpu().numpy()
+    b = ph.cu_phase_retrieval_holo(_t(i1, gpu), _t(i2, gpu), *arr, 5, n).data.cpu().numpy()
+    print(f"GS {n}^2: quadrant vs full tables, weighted phase RMS {asm.weighted_phase_rms(a, b):.2e} rad")
+    np.testing.assert_array_equal(a, b)
+
+
 def test_gs_arrays_vs_oracle(gpu):
     ph = _ph()
     n, z = 64, 2000.0
