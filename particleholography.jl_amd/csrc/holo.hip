@@ -425,13 +425,14 @@ int make_geom(holo_context* h, int nx, int ny, Geom* g, int* Ex, int* Ey, int ro
   if (rows * px->lp.T > 512 || rows * rowbytes + rowtw > maxl)
     return fail(HOLO_ERR_ARG, "row tile too large for a workgroup");
   g->rows = rows;
+  g->rtiles = (ny + rows - 1) / rows;
   // the reconstruction's slices in 4 x 4 blocks where pass A's tiles are at most 4 columns wide
   // (their row segments would be 32-byte pieces of 128-byte lines)
   g->gblk = (W <= h->gblk_w && nx % 4 == 0 && ny % 4 == 0 && px->lp.T % 4 == 0 && py->lp.T % 4 == 0) ? 1 : 0;
   return HOLO_OK;
 }
 
-dim3 row_grid(const Geom& g, int y = 1) { return dim3((g.ny + g.rows - 1) / g.rows, y); }
+dim3 row_grid(const Geom& g, int y = 1) { return dim3(g.rtiles, y); }
 dim3 row_block(const Geom& g) { return dim3(g.rows * g.px.T); }
 dim3 col_grid(const Geom& g, int y = 1) { return dim3((g.nx + g.W - 1) / g.W, y); }
 dim3 col_block(const Geom& g) { return dim3(g.W * g.py.T); }

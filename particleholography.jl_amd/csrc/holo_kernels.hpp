@@ -26,6 +26,7 @@ struct Geom {
   int W;        // columns per workgroup in column passes (power of two)
   int wshift;   // log2(W)
   int rows;     // rows per workgroup in row passes
+  int rtiles;   // row-pass workgroups along x: ceil(ny / rows) (the grid's x extent)
   int gblk;     // the reconstruction's column-transformed slices G are stored in 4 x 4 blocks
 };
 
@@ -236,7 +237,9 @@ __device__ __forceinline__ RowCtx row_ctx(const Geom& g, float2* lds) {
   const int T = line_threads<E, L>(g.px);
   c.t = threadIdx.x % T;
   c.w = threadIdx.x / T;
-  c.j = (g.gblk ? xcd_tile(blockIdx.x, gridDim.x) : static_cast<int>(blockIdx.x)) * g.rows + c.w;
+  // the tile count comes from the host: reading gridDim.x here makes the compiler fetch the
+  // workgroup size for a partial last workgroup with a vector load it must wait for
+  c.j = (g.gblk ? xcd_tile(blockIdx.x, g.rtiles) : static_cast<int>(blockIdx.x)) * g.rows + c.w;
   c.ok = c.j < g.ny;
   c.jl = c.ok ? c.j : g.ny - 1;
   c.line = lds + c.w * lds_padded_len(g.nx, kRowPad);
