@@ -1132,6 +1132,10 @@ int holo_create(holo_handle* out, int device) {
   // 1000 slices (pass A 56.7 -> 52.7 us per 32-slice launch), 1280x1024 2.52 -> 2.39 ms per 500;
   // with the twiddles from LDS instead (col_db 2) 3.19 and 2.45. Pass B at 1024^2 keeps both row
   // exchanges in LDS (row_pf 3: 46.2 against 46.9 us per launch with the last one by lane swaps).
+  // Pass A over 1024-tall columns holds one 512-thread workgroup per CU, so its workgroup count
+  // should fill whole waves of 256: at 1280 x 1024 the 160 eight-column tiles in 2 groups of 16
+  // slices made 320 (1.25 waves); rounds of 24 slices in groups of 8 make 480, and the
+  // projection's groups of 12 split the round evenly: 2.51-2.61 -> 2.29-2.35 ms per 500 slices.
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
@@ -1140,7 +1144,7 @@ int holo_create(holo_handle* out, int device) {
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
                                        {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 3, 3},
                                        {768, 768, 56, 16, 4, 4, 2},
-                                       {1280, 1024, 25, 16, 16, 8, 4, 0, 0, 3, 1},
+                                       {1280, 1024, 24, 8, 12, 8, 4, 0, 0, 3, 1},
                                        {2048, 2048, 64, 16, 16, 4, 2, 0, 0, -1}};
     for (const Shipped& e : kShipped) {
       Tuning t;
@@ -2117,14 +2121,16 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
     double ba = 1e30, bb = 1e30;
     Tuning ta = t, tb = t;
     // pass A (columns, generates H_k): columns per workgroup, slices per workgroup and one or
-    // two exchange buffers
+    // two exchange buffers. The round is cut to a multiple of the group, so that no workgroup
+    // runs a short remainder group (1280 x 1024: 25 slices in groups of 8 left one of 1).
     for (int W : {2, 4, 8, 16})
       for (int sg : {4, 8, 16})
         for (int db : {-1, 1, 2, 3}) {
           if (db >= 2 && !col_xs_form(ny)) continue;
           Tuning c = t;
           c.cols = W;
-          c.spg_gen = std::min(sg, t.batch);
+          if (t.batch > sg) c.batch = t.batch / sg * sg;
+          c.spg_gen = std::min(sg, c.batch);
           c.spg_out = 8;
           c.col_db = db;
           double v;
@@ -2133,12 +2139,14 @@ int holo_autotune(holo_handle h, void* stream, int nx, int ny, double wavlen, do
         }
     // pass B (rows, forms the projection): rows per workgroup, slices per workgroup and rows
     // loaded one slice ahead or not (with the twiddles in registers or in LDS)
+    // (half the round among the group sizes: the round in two equal groups)
     for (int rows : {1, 2, 4, 8})
-      for (int so : {4, 8, 16})
+      for (int so : {4, 8, 16, ta.batch / 2})
         for (int pf : {-1, 1, 2, 3}) {
+          if (so < 1 || (so == ta.batch / 2 && (so == 4 || so == 8 || so == 16))) continue;
           Tuning c = ta;
           c.rows = rows;
-          c.spg_out = std::min(so, t.batch);
+          c.spg_out = std::min(so, ta.batch);
           c.row_pf = pf;
           double v;
           if (time_round(c, false, true, &v) != HOLO_OK) continue;
