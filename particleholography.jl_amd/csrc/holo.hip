@@ -1136,6 +1136,11 @@ int holo_create(holo_handle* out, int device) {
   // should fill whole waves of 256: at 1280 x 1024 the 160 eight-column tiles in 2 groups of 16
   // slices made 320 (1.25 waves); rounds of 24 slices in groups of 8 make 480, and the
   // projection's groups of 12 split the round evenly: 2.51-2.61 -> 2.29-2.35 ms per 500 slices.
+  // 2048^2 keeps groups of 16 slices: one group per 64-slice round was faster alone (2.92 against
+  // 3.10-3.15 ms per 192 slices), but the serving slots run 16-slice rounds, and a request's
+  // result is the same in both only when their groups start at the same slices. Shorter rounds
+  // that would stay in the Infinity Cache lose more to launch tails (rounds of 8 or 16:
+  // 3.66-3.83 ms).
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
