@@ -1136,11 +1136,9 @@ int holo_create(holo_handle* out, int device) {
   // should fill whole waves of 256: at 1280 x 1024 the 160 eight-column tiles in 2 groups of 16
   // slices made 320 (1.25 waves); rounds of 24 slices in groups of 8 make 480, and the
   // projection's groups of 12 split the round evenly: 2.51-2.61 -> 2.29-2.35 ms per 500 slices.
-  // 2048^2 keeps groups of 16 slices: one group per 64-slice round was faster alone (2.92 against
-  // 3.10-3.15 ms per 192 slices), but the serving slots run 16-slice rounds, and a request's
-  // result is the same in both only when their groups start at the same slices. Shorter rounds
-  // that would stay in the Infinity Cache lose more to launch tails (rounds of 8 or 16:
-  // 3.66-3.83 ms).
+  // 2048^2 pass A generates H for its whole 64-slice round in one group (512 workgroups): 3.10-3.15
+  // -> 2.92 ms per 192 slices (groups of 32: 2.96-2.99). Shorter rounds that would stay in the
+  // Infinity Cache lose more to launch tails (rounds of 8 or 16: 3.66-3.83 ms).
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
@@ -1150,7 +1148,7 @@ int holo_create(holo_handle* out, int device) {
                                        {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 3, 3},
                                        {768, 768, 56, 16, 4, 4, 2},
                                        {1280, 1024, 24, 8, 12, 8, 4, 0, 0, 3, 1},
-                                       {2048, 2048, 64, 16, 16, 4, 2, 0, 0, -1}};
+                                       {2048, 2048, 64, 64, 16, 4, 2, 0, 0, -1}};
     for (const Shipped& e : kShipped) {
       Tuning t;
       t.batch = e.batch;

@@ -121,12 +121,12 @@ def select_batch(waiting: deque, used: int, budget: int, max_batch: int, others_
     return batch
 
 
-# Round sizes of the slot handles where they differ from a lone stream's. A lone 2048^2
-# reconstruction runs 64-slice rounds to amortise each launch's tail; with other slots in flight
-# they fill those tails, and 16-slice rounds serve more (capacity of the 1000-request stream on
-# MI355X, 1024^2-equivalent slices/s: four slots 171k against 159k with 64-slice rounds, two
-# slots 176k against 172k).
-SLOT_TUNING = {(2048, 2048): (16, 16, 4, 4, 2)}  # batch, spg_gen, spg_out, cols, rows
+# Round sizes of the slot handles where they differ from a lone stream's (batch, spg_gen,
+# spg_out, cols, rows). None at present: with the blocked intermediate slices the slots serve
+# more with the lone tiles (capacity of the 1500-request stream on MI355X, 1024^2-equivalent
+# slices/s: 214.9k against 208.2k with 16-slice rounds at 2048^2), and equal tiles keep a served
+# result bitwise equal to the same request alone.
+SLOT_TUNING: dict = {}
 
 
 class Server:
@@ -135,7 +135,7 @@ class Server:
     # 15.2 / 113 ms. Fewer slots share the device among fewer groups, so the 2048^2 requests
     # that make the tail finish sooner; small requests wait a little longer.
     def __init__(self, device: int = 0, n_slots: int = 2, max_batch: int = 32,
-                 hbm_budget: int = 32 << 30, workspace_per_slot: int = 2 << 30,
+                 hbm_budget: int = 32 << 30, workspace_per_slot: int = 4 << 30,
                  max_group_work: Optional[int] = 32 << 30, keep_results: bool = True):
         # keep_results: each request keeps its host projection (Request.proj); False hands the
         # result over at completion and releases its pinned host buffer for reuse, so that a

@@ -61,7 +61,9 @@ def test_c5_mix_served_equals_alone_and_fp64_oracle(gpu):
     from particleholography import serve
 
     reqs = serve.make_stream(16, rate=0.0, seed=105, pool=1, gs_fraction=0.35)
-    srv = serve.Server(n_slots=2, max_batch=8, hbm_budget=32 << 30, workspace_per_slot=2 << 30)
+    # each slot's workspace holds a 2048^2 round (spectrum + 64 slices, 2.03 GiB), so served rounds
+    # are the lone call's and the results are equal bit for bit
+    srv = serve.Server(n_slots=2, max_batch=8, hbm_budget=32 << 30, workspace_per_slot=4 << 30)
     stats = srv.run(reqs)
     assert stats["requests"] == 16 and set(stats["per_size"]) <= {"512x512", "768x768", "1024x1024",
                                                                     "1280x1024", "2048x2048"}
