@@ -1138,7 +1138,9 @@ int holo_create(holo_handle* out, int device) {
   // projection's groups of 12 split the round evenly: 2.51-2.61 -> 2.29-2.35 ms per 500 slices.
   // 2048^2 pass A generates H for its whole 64-slice round in one group (512 workgroups): 3.10-3.15
   // -> 2.92 ms per 192 slices (groups of 32: 2.96-2.99). Shorter rounds that would stay in the
-  // Infinity Cache lose more to launch tails (rounds of 8 or 16: 3.66-3.83 ms).
+  // Infinity Cache lose more to launch tails (rounds of 8 or 16: 3.66-3.83 ms). 768^2 reads its
+  // blocked slices in 4-row tiles (whole blocks: pass B 0.53 -> 0.49-0.50 ms per 500 slices) and
+  // runs rounds of 48 in four full groups of 12: 1.09-1.20 -> 1.04-1.05 ms per 500 slices.
   // holo_set_tuning_for_size / holo_autotune replace them.
   {
     struct Shipped {
@@ -1146,7 +1148,7 @@ int holo_create(holo_handle* out, int device) {
     };
     static const Shipped kShipped[] = {{512, 512, 0, 0, 0, 8, 0},
                                        {1024, 1024, 0, 0, 0, 0, 0, 0, 0, 3, 3},
-                                       {768, 768, 56, 16, 4, 4, 2},
+                                       {768, 768, 48, 12, 4, 4, 4},
                                        {1280, 1024, 24, 8, 12, 8, 4, 0, 0, 3, 1},
                                        {2048, 2048, 64, 64, 16, 4, 2, 0, 0, -1}};
     for (const Shipped& e : kShipped) {
