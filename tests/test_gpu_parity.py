@@ -196,6 +196,43 @@ def test_pass_a_exchange_variants_bitwise_equal(gpu, nx, ny):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("n", [768, 2048])
+def test_blocked_and_row_major_intermediate_slices_bitwise_equal(gpu, n, monkeypatch):
+    """768^2 and 2048^2 run pass A on 4-column tiles, whose intermediate slices are stored in 4 x 4
+    blocks; a handle created with HOLO_GBLK_W=0 stores them row-major with the same tiles. The
+    projection and a Float32 volume with its projection (the store pass reads the slices too)
+    are the same bit for bit."""
+    import ctypes
+
+    from particleholography import _lib
+
+    slices = 19
+    h = _holo(n, 23)
+    wf = _t(np.sqrt(h).astype(np.complex64), gpu)
+    lib = _lib.load()
+    f = _lib.holo_tf(None, -1.5e4, LAM, DX, 1)
+    d = _lib.holo_tf(None, -275.0, LAM, DX, 1)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    monkeypatch.setenv("HOLO_GBLK_W", "0")
+    row_major = _lib.Handle(0)  # the layout knob is read when a handle is created
+    monkeypatch.delenv("HOLO_GBLK_W")
+    outs = []
+    for hd in (_lib.handle(0), row_major):
+        proj = torch.empty((n, n), dtype=torch.float32, device=gpu)
+        vproj = torch.empty((n, n), dtype=torch.float32, device=gpu)
+        vol = torch.empty((slices, n, n), dtype=torch.float32, device=gpu)
+        _lib.check(lib.holo_reconst_xyprojection(hd.ptr, st, n, n, ctypes.c_void_p(wf.data_ptr()), ctypes.byref(f),
+                                                 ctypes.byref(d), slices, ctypes.c_void_p(proj.data_ptr())))
+        _lib.check(lib.holo_reconst_vol(hd.ptr, st, n, n, ctypes.c_void_p(wf.data_ptr()), ctypes.byref(f),
+                                        ctypes.byref(d), slices, 2, ctypes.c_void_p(vol.data_ptr()),
+                                        ctypes.c_void_p(vproj.data_ptr())))
+        torch.cuda.synchronize()
+        outs.append((proj.cpu().numpy(), vol.cpu().numpy(), vproj.cpu().numpy()))
+    for a, b, what in zip(outs[0], outs[1], ("projection", "volume", "volume projection")):
+        assert np.array_equal(a, b), f"{n}^2 {what}: blocked and row-major intermediate slices differ"
+    print(f"{n}^2 x {slices}: blocked == row-major intermediate slices (projection, volume)")
+
+
 @pytest.mark.parametrize("nx,ny", [(80, 64), (64, 96), (160, 48)])
 def test_projection_non_square(gpu, nx, ny):
     from particleholography import _lib
