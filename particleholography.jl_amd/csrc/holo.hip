@@ -1838,6 +1838,10 @@ int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* j
     Tuning gt = plane_tiles(h, nx, ny);
     gt.erow = 0;
     gt.rows = h->tune.rows;
+    // 2048-tall columns take 2-column tiles here: 16 pairs x 20 iterations, 10.9k -> 11.25k
+    // pair-iterations/s (at 1024^2 they cost the lane-swap columns 20%); a tile's width only
+    // chooses which columns a workgroup transforms, so the results are the same
+    if (nx == 2048 && ny == 2048 && h->tune.cols <= 0) gt.cols = 2;
     HOLO_TRY(make_geom(h, nx, ny, &g, &Ex, &Ey, 256, &gt));
     const size_t plane = static_cast<size_t>(nx) * ny;
     // per pair: field, forward and backward tables, the amplitudes sqrt(I1), sqrt(I2) (so that

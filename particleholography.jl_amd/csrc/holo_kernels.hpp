@@ -510,10 +510,10 @@ __global__ __launch_bounds__(512) void k_recon_cols(RecArgs a, JobTable jt, Geom
         float2 v[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = f[e];
-        if (k + 1 < k1) {
+        // unconditional (the group's last rotation is unused): a branch here made the compiler
+        // merge two versions of f with a register copy per element and slice
 #pragma unroll
-          for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
-        }
+        for (int e = 0; e < E; ++e) f[e] = cmul(f[e], r[e]);
         fft_col_xs<E, +1, L, TWR>(v, c.col + ((k & 1) ? bo : 0), W, c.t, c.tw, twr);
         col_store_buf(v, plane_rsrc(G + static_cast<size_t>(k) * a.gstride, pbytes), ok ? offg * 8u : pbytes, rowb);
       }
