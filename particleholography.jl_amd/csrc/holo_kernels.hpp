@@ -33,7 +33,7 @@ struct Geom {
 // Element (row j, column i) of a reconstruction slice G: natural order j * nx + i, or, with
 // gblk, in blocks of 4 rows x 4 columns (16 consecutive elements, one 128-byte line), so that a
 // column pass over 4-column tiles writes whole lines instead of 32-byte pieces of four
-// (2048^2 pass A: 2.1 -> 1.6 ms per 192 slices). Needs nx and ny multiples of 4, and threads
+// (2048^2 pass A: 2.1 -> 1.5 ms per 192 slices). Needs nx and ny multiples of 4, and threads
 // per line (both passes) multiples of 4: element t + T*e of a line then sits at the thread's
 // base plus T*nx*e (columns) or 4*T*e (rows).
 __device__ __forceinline__ unsigned g_index(const Geom& g, unsigned j, unsigned i) {
