@@ -465,10 +465,12 @@ def _gs_case(n, seed, z=2000.0):
     return i1, i2
 
 
-@pytest.mark.parametrize("n,iters", [(64, 1), (64, 10), (128, 25), (512, 20), (1024, 20), (2048, 3)])
+@pytest.mark.parametrize("n,iters", [(64, 1), (64, 10), (128, 25), (512, 20), (1024, 20), (2048, 3), (5, 6)])
 def test_gs_generated_vs_fp64_oracle(gpu, n, iters):
     """Small sides on runtime plans; 512^2, 1024^2 and 2048^2 on the single-plane line shapes
-    and their compile-time plans (20 iterations: one 16-iteration graph plus a remainder)."""
+    and their compile-time plans (20 iterations: one 16-iteration graph plus a remainder). Even
+    sides read their generated transfers as quadrant tables; the odd side 5 (odd supported sides
+    are 3 and 5) reads the whole table."""
     ph = _ph()
     z = 2000.0
     i1, i2 = _gs_case(n, n + iters, z)
