@@ -52,3 +52,28 @@ def test_work_cap_closes_groups_but_admits_one():
     assert _ids(select_batch(w, 0, 1000, 8, False, max_work=64)) == [0, 1]   # 30 + 30 <= 64 < 110
     assert _ids(select_batch(w, 0, 1000, 8, False, max_work=40)) == [2]      # first one always fits
     assert _ids(select_batch(w, 0, 1000, 8, False, max_work=40)) == [3]
+
+
+class _S(_W):
+    def __init__(self, rid, fp, work, size):
+        super().__init__(rid, fp, work)
+        self.size = size
+
+
+def test_same_size_lookahead_groups_one_size_and_keeps_the_others_in_order():
+    w = deque(_S(i, 1, 10, s) for i, s in enumerate(["a", "b", "a", "c", "a", "b", "a"]))
+    key = lambda r: r.size  # noqa: E731
+    # the oldest request and the same-size ones within the window of 5 (ids 0, 2, 4; 6 is beyond)
+    assert _ids(select_batch(w, 0, 1000, 8, False, lookahead=5, key=key)) == [0, 2, 4]
+    assert _ids(w) == [1, 3, 5, 6]
+    assert _ids(select_batch(w, 0, 1000, 8, False, lookahead=5, key=key)) == [1, 5]
+    assert _ids(w) == [3, 6]
+    # limits still apply: max_batch, the work cap (the first request always fits), the budget
+    w = deque(_S(i, 1, 30, "a") for i in range(6))
+    assert _ids(select_batch(w, 0, 1000, 2, False, lookahead=8, key=key)) == [0, 1]
+    assert _ids(select_batch(w, 0, 1000, 8, False, max_work=64, lookahead=8, key=key)) == [2, 3]
+    assert _ids(select_batch(w, 998, 1000, 8, False, lookahead=8, key=key)) == [4, 5]
+    # the head waits when it does not fit and other work is in flight
+    w = deque([_S(0, 50, 1, "a"), _S(1, 1, 1, "a")])
+    assert _ids(select_batch(w, 60, 100, 8, True, lookahead=8, key=key)) == []
+    assert _ids(w) == [0, 1]
