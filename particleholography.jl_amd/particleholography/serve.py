@@ -170,6 +170,15 @@ def _select_same_key(waiting, used, budget, max_batch, others_in_flight, max_wor
 SLOT_TUNING: dict = {}
 
 
+def group_work_cap(max_work: Optional[int], waiting_work: int, n_slots: int) -> Optional[int]:
+    """The per-group work cap, grown with the queue: when the waiting work exceeds eight groups
+    of the cap per slot, latency is set by queueing, and larger groups (more holograms and GS
+    pairs per launch) drain the queue sooner. None: no cap."""
+    if max_work is None:
+        return None
+    return max(max_work, waiting_work // (8 * n_slots))
+
+
 def _size_key(r: Request):
     return tuple(r.holo.shape)
 
@@ -261,12 +270,7 @@ class Server:
                     inflight.remove(g)
             free = [s for s in self.slots if not s.busy]
             if waiting and free:
-                # the work cap grows with the queue: while the waiting work exceeds what the slots
-                # would take in groups of the cap, latency is set by queueing, and larger groups
-                # (more holograms and GS pairs per launch) drain it sooner
-                cap = self.max_group_work
-                if cap is not None:
-                    cap = max(cap, waiting_work // (8 * len(self.slots)))
+                cap = group_work_cap(self.max_group_work, waiting_work, len(self.slots))
                 batch = select_batch(waiting, used, self.budget, self.max_batch, bool(inflight), cap,
                                      self.lookahead, _size_key)
                 waiting_work -= sum(r.work_bytes() for r in batch)

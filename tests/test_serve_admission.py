@@ -1,7 +1,7 @@
 """Admission rule of the serving loop, with stand-in requests (CPU only)."""
 from collections import deque
 
-from particleholography.serve import select_batch
+from particleholography.serve import group_work_cap, select_batch
 
 
 class _R:
@@ -77,3 +77,10 @@ def test_same_size_lookahead_groups_one_size_and_keeps_the_others_in_order():
     w = deque([_S(0, 50, 1, "a"), _S(1, 1, 1, "a")])
     assert _ids(select_batch(w, 60, 100, 8, True, lookahead=8, key=key)) == []
     assert _ids(w) == [0, 1]
+
+
+def test_group_work_cap_grows_with_the_queue_only():
+    assert group_work_cap(None, 10 ** 15, 2) is None
+    assert group_work_cap(32, 0, 2) == 32            # short queue: the configured cap
+    assert group_work_cap(32, 16 * 32, 2) == 32      # exactly eight groups per slot: unchanged
+    assert group_work_cap(32, 16 * 320, 2) == 320    # a deep queue: an eighth of it per slot
