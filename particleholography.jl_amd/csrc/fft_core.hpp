@@ -263,8 +263,9 @@ struct DFT<20, DIR> {
 // writes across banks. Rows (one line per thread group, stride 1) use one slot in 32: their
 // radix-4 stages of the 768 and 1280 plans write with element stride 4, which one slot in 16
 // maps onto repeating banks (measured on the 1280 row pass: 11.1M bank-conflict cycles per
-// launch with 1/16, 6.0M with 1/32, 12% shorter). Columns (W interleaved lines) keep one in
-// 16, under which the 1024 column pass has no conflicts at all.
+// launch with 1/16, 6.0M with 1/32, 12% shorter). Sparser slots did not help it (1/64 and
+// 1/128: the same 1280 row pass) and cost the 1024 one 4% and 30%. Columns (W interleaved lines)
+// keep one in 16, under which the 1024 column pass has no conflicts at all.
 // ------------------------------------------------------------------------------------------
 constexpr int kRowPad = 5;
 constexpr int kColPad = 4;
