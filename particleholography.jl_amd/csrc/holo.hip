@@ -119,7 +119,7 @@ struct holo_context {
   int col_e = 0;                 // elements per thread of column passes (0: the length's default)
   int col_db = 1;                // pass A exchange buffers: HOLO_COL_DB 0 one, 1 per size, 2 two
   int row_pf = 1;                // pass B rows one slice ahead: HOLO_ROW_PF 0 never, 1 per size, 2 always
-  int gs_xs = 1;                 // single-plane column passes over 1024-tall columns: HOLO_GS_XS 0 the
+  int gs_xs = 1;                 // batched GS column passes over 1024-tall columns: HOLO_GS_XS 0 the
                                  // plane tiles' 8 elements per thread, 1 / 2 16 elements with lane-swap
                                  // exchanges and one / two LDS buffers
   int gblk_w = 4;                // widest pass-A tile whose slices are stored in 4 x 4 blocks
@@ -338,15 +338,16 @@ Tuning eff_tuning(const holo_context* h, int nx, int ny) {
 // per thread (more waves, one more LDS exchange) and narrower tiles. Measured on MI355X (E rows / columns, iterations/s): 1024^2 E 4/8 on 4-column and
 // 1-row tiles 20.9k (16.2k at the reconstruction's E 16/16, 20.3k at E 8/8); 512^2 E 4/8, 4 x 1
 // 42.0k; 2048^2 E 8/16, 4 x 1 8.7k. Global tuning (holo_set_tuning) still sets the tile sizes.
-// 1024-tall columns take 16 elements per thread with the lane-swap exchanges (HOLO_GS_XS 1, the
-// default): each transform's last exchange in registers, three barriers per column pass instead
-// of twelve at 8 elements; batched GS 1024^2 x 16 pairs 53.1k -> 56.6k pair-iterations/s (32
-// pairs 52.2k -> 53.8k), single pairs and cu_asm_prop! unchanged.
-Tuning plane_tiles(const holo_context* h, int nx, int ny) {
+// In batched GS, 1024-tall columns take 16 elements per thread with the lane-swap exchanges
+// (HOLO_GS_XS 1, the default): each transform's last exchange in registers, three barriers per
+// column pass instead of twelve at 8 elements; 1024^2 x 16 pairs 53.1k -> 56.6k
+// pair-iterations/s. A single plane keeps 8 (a single GS pair 21.4k against 20.75k iterations/s
+// with the lane-swap form; cu_asm_prop! the same either way).
+Tuning plane_tiles(const holo_context* h, int nx, int ny, bool batched_gs = false) {
   Tuning t;
   if (nx == 1024 && ny == 1024) {
     t.erow = 4;
-    t.ecol = h->gs_xs > 0 ? 0 : 8;
+    t.ecol = batched_gs && h->gs_xs > 0 ? 0 : 8;
     t.cols = 4;
     t.rows = 1;
   } else if (nx == 512 && ny == 512) {
@@ -1835,7 +1836,7 @@ int holo_phase_retrieval_batch(holo_handle h, void* stream, const holo_gs_job* j
     // single-pair call's five), columns keep the single-plane tiles. 1024^2 x 16 pairs: 50.4k ->
     // 53.2k pair-iterations/s (all-default tiles 47.9k). The tiles depend on the size only, so a
     // pair's result does not depend on what it is batched with.
-    Tuning gt = plane_tiles(h, nx, ny);
+    Tuning gt = plane_tiles(h, nx, ny, true);
     gt.erow = 0;
     gt.rows = h->tune.rows;
     // 2048-tall columns take 2-column tiles here: 16 pairs x 20 iterations, 10.9k -> 11.25k
