@@ -115,6 +115,8 @@ def test_autotune_keeps_results_correct(gpu):
     n = 256
     choice = batch.autotune(n, n, LAM, DX, save=False)
     assert choice["batch"] >= 1 and choice["cols_per_wg"] in (1, 2, 4, 8, 16)
+    # rounds are whole numbers of generation groups: no workgroup runs a short remainder group
+    assert choice["batch"] % choice["spg_gen"] == 0 or choice["batch"] < choice["spg_gen"], choice
     eff = batch.get_tuning_for_size(n, n, 1000)
     assert eff["batch"] == choice["batch"] and eff["spg_gen"] == choice["spg_gen"]
     # the reported choice is the tiles in force, defaults resolved (also when the incumbent,
