@@ -244,3 +244,19 @@ def test_workspace_size_is_queryable(gpu):
 
     rounds = get_tuning_for_size(n, n, slices)["batch"]
     assert h.workspace_size() >= (1 + rounds) * n * n * 8
+
+
+def test_gs_batch_chunks_leave_each_pair_unchanged(gpu):
+    """At 1024^2 a batched GS call runs 16 pairs per launch (their fields and amplitudes fill
+    256 MiB): 18 pairs run as chunks of 16 and 2, and every result equals the pair computed
+    alone through the same entry point, bit for bit."""
+    from particleholography.batch import GSJob, phase_retrieval_batch
+
+    n, z, iters = 1024, 2000.0, 3
+    pairs = [_gs_pair(n, 300 + k % 3, z) for k in range(3)]
+    ts = [(torch.from_numpy(a).to(gpu), torch.from_numpy(b).to(gpu)) for a, b in pairs]
+    jobs = [GSJob(ts[k % 3][0], ts[k % 3][1], z + 10.0 * k, iters, LAM, DX) for k in range(18)]
+    outs = phase_retrieval_batch(jobs)
+    for k in (0, 7, 15, 16, 17):
+        alone = phase_retrieval_batch([jobs[k]])[0]
+        np.testing.assert_array_equal(outs[k].cpu().numpy(), alone.cpu().numpy(), err_msg=f"pair {k}")
