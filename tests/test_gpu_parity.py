@@ -233,6 +233,42 @@ def test_blocked_and_row_major_intermediate_slices_bitwise_equal(gpu, n, monkeyp
     print(f"{n}^2 x {slices}: blocked == row-major intermediate slices (projection, volume)")
 
 
+@pytest.mark.parametrize("nx,ny", [(768, 512), (512, 768)])
+def test_blocked_layout_non_square(gpu, nx, ny):
+    """Non-square planes whose sides are multiples of 4: 4-column pass-A tiles store the
+    intermediate slices in 4 x 4 blocks (nx / 4 blocks per block row), 8-column tiles row-major;
+    the projections are bitwise equal and within the bar of the FP64 oracle."""
+    import ctypes
+
+    from particleholography import _lib
+
+    slices = 21
+    h = _holo(nx, 29, ny)
+    ht = _t(h, gpu)
+    lib = _lib.load()
+    f = _lib.holo_tf(None, -1.2e4, LAM, DX, 1)
+    d = _lib.holo_tf(None, -410.0, LAM, DX, 1)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    hd = _lib.handle(0)
+    outs = {}
+    try:
+        for w in (4, 8):
+            hd.set_tuning(0, 0, 0, w, 0)
+            proj = torch.empty((ny, nx), dtype=torch.float32, device=gpu)
+            _lib.check(lib.holo_reconst_xyprojection_holo(hd.ptr, st, nx, ny, ctypes.c_void_p(ht.data_ptr()),
+                                                           ctypes.byref(f), ctypes.byref(d), slices,
+                                                           ctypes.c_void_p(proj.data_ptr())))
+            outs[w] = proj.cpu().numpy()
+    finally:
+        hd.set_tuning()
+    assert np.array_equal(outs[4], outs[8])
+    want = asm.reconst_xyprojection(asm.gabor_wavefront(h), slices, "exact", z_front=-1.2e4, z_dz=-410.0,
+                                    wavlen=LAM, dx=DX)
+    err = asm.rel_l2(outs[4], want)
+    print(f"{nx}x{ny} x {slices}: blocked == row-major, {err:.2e} rel L2 vs FP64 oracle")
+    assert err < 1e-4
+
+
 @pytest.mark.parametrize("nx,ny", [(80, 64), (64, 96), (160, 48)])
 def test_projection_non_square(gpu, nx, ny):
     from particleholography import _lib
