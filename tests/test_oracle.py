@@ -35,6 +35,22 @@ def test_transfer_sqrt_centre_and_symmetry():
 # ---------------------------------------------------------------------------------------
 # A3: transfer functions
 # ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("nx,ny", [(8, 8), (64, 64), (1024, 1024), (768, 512), (512, 768)])
+def test_natural_transfer_is_determined_by_its_quadrant(nx, ny):
+    """For even sides the natural-order transfer depends on each axis through the square of
+    (ic - N/2), and natural indices n and N - n carry opposite values of it, exactly in FP64:
+    H(i, j) = H(min(i, nx - i), min(j, ny - j)). The GS column pass reads generated transfers as
+    that quadrant of (nx/2 + 1) x (ny/2 + 1) entries; rebuilt by its index rule, it is the whole
+    table bit for bit."""
+    h = asm.natural(asm.transfer_exact(2000.0, 0.6328, 10.0, nx, ny))
+    q = h[: ny // 2 + 1, : nx // 2 + 1]
+    j = np.minimum(np.arange(ny), ny - np.arange(ny))
+    i = np.minimum(np.arange(nx), nx - np.arange(nx))
+    assert np.array_equal(q[j[:, None], i[None, :]], h)
+    # an asymmetric mistake (one index off) is caught
+    assert not np.array_equal(q[np.minimum(j + 1, ny // 2)[:, None], i[None, :]], h)
+
+
 def test_transfer_reference_properties():
     n = 64
     d32 = asm.transfer_sqrt_arr(n, LAM, DX)
